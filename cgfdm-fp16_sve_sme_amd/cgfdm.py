@@ -1,0 +1,214 @@
+"""Python binding of the MI355X CGFDM C ABI (include/cgfdm.h) over ctypes.
+
+The library (libcgfdm.so, built from csrc/ by build.py) holds every kernel; this module only
+marshals host arrays. There is no fallback: if the HIP library is missing or cannot be
+loaded, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcgfdm.so")
+
+FP32, FP16, BF16 = 1, 2, 3
+PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}
+
+STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECFL", -5: "EPML",
+          -6: "ESTATE", -7: "EIO"}
+
+
+class Desc(ctypes.Structure):
+    """Mirror of struct cgfdm_desc (include/cgfdm.h)."""
+    _fields_ = [
+        ("NX", ctypes.c_int), ("NY", ctypes.c_int), ("NZ", ctypes.c_int),
+        ("nPML", ctypes.c_int),
+        ("DH", ctypes.c_double), ("DT", ctypes.c_double),
+        ("NT", ctypes.c_int), ("precision", ctypes.c_int),
+        ("free_surface", ctypes.c_int), ("pml", ctypes.c_int),
+        ("use_terrain", ctypes.c_int),
+        ("depth", ctypes.c_double),
+        ("centerX", ctypes.c_int), ("centerY", ctypes.c_int),
+        ("Cv", ctypes.c_double), ("Cs", ctypes.c_double), ("Crho", ctypes.c_double),
+        ("check_cfl", ctypes.c_int),
+    ]
+
+
+def make_desc(NX, NY, NZ, DH=100.0, DT=0.008, NT=100, nPML=12, precision=FP32,
+              free_surface=1, pml=1, use_terrain=0, depth=None, centerX=None, centerY=None,
+              Cv=0.0, Cs=0.0, Crho=0.0, check_cfl=0):
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    d = Desc()
+    d.NX, d.NY, d.NZ, d.nPML = NX, NY, NZ, nPML
+    d.DH, d.DT, d.NT, d.precision = DH, DT, NT, precision
+    d.free_surface, d.pml, d.use_terrain = free_surface, pml, use_terrain
+    d.depth = DH * NZ if depth is None else depth
+    d.centerX = NX // 2 if centerX is None else centerX
+    d.centerY = NY // 2 if centerY is None else centerY
+    d.Cv, d.Cs, d.Crho, d.check_cfl = Cv, Cs, Crho, check_cfl
+    return d
+
+
+def default_scales(precision):
+    """Reference scale factors (macro.h:65-73): 1 for FP32, (5e3, 1e-6, 1e6) otherwise."""
+    if precision == FP32:
+        return 1.0, 1.0, 1.0
+    return 5.0e3, 1.0e-6, 1.0e6
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, ll, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+        P = ctypes.POINTER
+        L.cgfdm_last_error.restype = ctypes.c_char_p
+        L.cgfdm_domain_create.argtypes = [vp, i, P(vp)]
+        L.cgfdm_domain_destroy.argtypes = [vp]
+        L.cgfdm_domain_bytes.argtypes = [vp]
+        L.cgfdm_domain_bytes.restype = ll
+        L.cgfdm_estimate_bytes.argtypes = [vp]
+        L.cgfdm_estimate_bytes.restype = ll
+        L.cgfdm_set_model.argtypes = [vp, vp, vp, vp, vp]
+        L.cgfdm_set_source_point.argtypes = [vp, i, i, i, f]
+        L.cgfdm_set_source_table.argtypes = [vp, vp, vp, ll, i, f]
+        L.cgfdm_set_stations.argtypes = [vp, vp, i]
+        L.cgfdm_step.argtypes = [vp, i]
+        L.cgfdm_sync.argtypes = [vp]
+        L.cgfdm_current_step.argtypes = [vp]
+        for n in ("cgfdm_get_stations", "cgfdm_get_pgv", "cgfdm_get_state", "cgfdm_set_state",
+                  "cgfdm_get_cjm"):
+            getattr(L, n).argtypes = [vp, vp]
+        L.cgfdm_get_slice.argtypes = [vp, i, i, i, vp]
+        L.cgfdm_stage_timing.argtypes = [vp, i, P(ctypes.c_double), P(ll)]
+        L.cgfdm_enable_timing.argtypes = [vp, i]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class CgfdmError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc != 0:
+        raise CgfdmError(f"cgfdm error {STATUS.get(rc, rc)}: {lib().cgfdm_last_error().decode()}")
+
+
+class Domain:
+    """One simulation on one GPU (a cgfdm_domain handle)."""
+
+    def __init__(self, desc, device=0):
+        self.desc = desc
+        h = ctypes.c_void_p()
+        _chk(lib().cgfdm_domain_create(ctypes.byref(desc), device, ctypes.byref(h)))
+        self.h = h
+        self.nx_, self.ny_, self.nz_ = desc.NX + 6, desc.NY + 6, desc.NZ + 6
+        self.num = self.nx_ * self.ny_ * self.nz_
+        self.nst = 0
+
+    def close(self):
+        if self.h:
+            lib().cgfdm_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def bytes(self):
+        return lib().cgfdm_domain_bytes(self.h)
+
+    def set_model(self, terrain, vs, vp, rho):
+        t = None if terrain is None else np.ascontiguousarray(terrain, dtype=np.float64)
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (vs, vp, rho)]
+        _chk(lib().cgfdm_set_model(self.h, _p(t), _p(a[0]), _p(a[1]), _p(a[2])))
+
+    def set_source_point(self, X, Y, Z, fc):
+        _chk(lib().cgfdm_set_source_point(self.h, X, Y, Z, fc))
+
+    def set_source_table(self, xyz, rate, dt):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        rate = np.ascontiguousarray(rate, dtype=np.float32)
+        nt, npts = rate.shape[0], rate.shape[1]
+        _chk(lib().cgfdm_set_source_table(self.h, _p(xyz), _p(rate), npts, nt, dt))
+
+    def set_stations(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        self.nst = xyz.shape[0]
+        _chk(lib().cgfdm_set_stations(self.h, _p(xyz), self.nst))
+
+    def step(self, n=1):
+        _chk(lib().cgfdm_step(self.h, n))
+
+    def sync(self):
+        _chk(lib().cgfdm_sync(self.h))
+
+    @property
+    def it(self):
+        return lib().cgfdm_current_step(self.h)
+
+    def stations(self):
+        out = np.zeros((self.nst, self.desc.NT, 9), np.float32)
+        _chk(lib().cgfdm_get_stations(self.h, _p(out)))
+        return out
+
+    def pgv(self):
+        out = np.zeros((self.desc.NY, self.desc.NX, 2), np.float32)
+        _chk(lib().cgfdm_get_pgv(self.h, _p(out)))
+        return out
+
+    def slice(self, axis, index, var):
+        d = self.desc
+        shape = {0: (d.NZ, d.NY), 1: (d.NZ, d.NX), 2: (d.NY, d.NX)}[axis]
+        out = np.zeros(shape, np.float32)
+        _chk(lib().cgfdm_get_slice(self.h, axis, index, var, _p(out)))
+        return out
+
+    def state(self):
+        d = self.desc
+        out = np.zeros((9, d.NZ, d.NY, d.NX), np.float32)
+        _chk(lib().cgfdm_get_state(self.h, _p(out)))
+        return out
+
+    def set_state(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _chk(lib().cgfdm_set_state(self.h, _p(w)))
+
+    def cjm(self):
+        out = np.zeros((13, self.nz_, self.ny_, self.nx_), np.float32)
+        _chk(lib().cgfdm_get_cjm(self.h, _p(out)))
+        return out
+
+    def enable_timing(self, on=True):
+        _chk(lib().cgfdm_enable_timing(self.h, 1 if on else 0))
+
+    def stage_timing(self, reset=False):
+        avg = ctypes.c_double()
+        cnt = ctypes.c_longlong()
+        _chk(lib().cgfdm_stage_timing(self.h, 1 if reset else 0, ctypes.byref(avg), ctypes.byref(cnt)))
+        return avg.value, cnt.value
+
+
+def estimate_bytes(desc):
+    return lib().cgfdm_estimate_bytes(ctypes.byref(desc))

@@ -1,0 +1,245 @@
+// setup.hip — FP64 construction of the packed metric/Jacobian/medium array on the GPU.
+//
+// Coordinates are never stored: the reference builds them as x = (I-3-centerX)*DH,
+// y = (J-3-centerY)*DH and either a flat z (coord.cpp:106-142) or the terrain-following
+// stretch z = -|D| + (h+|D|)/(NZ-1)*(K-3) (coord.cpp:255-307, terrain.cpp:930-937), so each
+// thread evaluates the coordinates of its stencil neighbours on the fly from the 2-D
+// height map. All arithmetic is FP64; the result is rounded once to the storage type.
+#include "common.hpp"
+
+namespace cg {
+
+__device__ __forceinline__ void coord_at(const SetupParams& q, int i, int j, int k, double c[3]) {
+    c[0] = (i - HALO) * q.DH - q.centerX * q.DH;
+    c[1] = (j - HALO) * q.DH - q.centerY * q.DH;
+    if (q.use_terrain) {
+        double h = q.terrain[i + (long long)j * q.nx_];
+        double D = fabs(q.depth);
+        double DZ = (h + D) / double(q.nz - 1);
+        c[2] = -D + DZ * (k - HALO);
+    } else {
+        c[2] = (k - HALO + 1) * q.DH - q.nz * q.DH;
+    }
+}
+
+// Averaged forward/backward MacCormack derivative of the coordinates and the contravariant
+// metric as the adjugate over J (reference solve_con_jac, contravariant.cpp:71-171).
+__device__ void metric_at(const SetupParams& q, int i, int j, int k, double m[9], double& J) {
+    const double rDH = 1.0 / q.DH;
+    const double a[5] = {(double)af_1, (double)af0, (double)af1, (double)af2, (double)af3};
+    double der[3][3];  // der[var][dir]
+    for (int dir = 0; dir < 3; ++dir) {
+        double cp[5][3], cm[5][3];
+        for (int s = 0; s < 5; ++s) {
+            int o = s - 1;  // offsets -1..3
+            int ip = i + (dir == 0 ? o : 0), jp = j + (dir == 1 ? o : 0), kp = k + (dir == 2 ? o : 0);
+            int im = i - (dir == 0 ? o : 0), jm = j - (dir == 1 ? o : 0), km = k - (dir == 2 ? o : 0);
+            coord_at(q, ip, jp, kp, cp[s]);
+            coord_at(q, im, jm, km, cm[s]);
+        }
+        for (int v = 0; v < 3; ++v) {
+            double lp = (a[0] * cp[0][v] + a[1] * cp[1][v] + a[2] * cp[2][v] + a[3] * cp[3][v] + a[4] * cp[4][v]) * rDH;
+            double lm = -(a[0] * cm[0][v] + a[1] * cm[1][v] + a[2] * cm[2][v] + a[3] * cm[3][v] + a[4] * cm[4][v]) * rDH;
+            der[v][dir] = 0.5 * (lp + lm);
+        }
+    }
+    const double x_xi = der[0][0], x_et = der[0][1], x_zt = der[0][2];
+    const double y_xi = der[1][0], y_et = der[1][1], y_zt = der[1][2];
+    const double z_xi = der[2][0], z_et = der[2][1], z_zt = der[2][2];
+    J = x_xi * y_et * z_zt + x_et * y_zt * z_xi + x_zt * y_xi * z_et - x_zt * y_et * z_xi - x_et * y_xi * z_zt -
+        x_xi * z_et * y_zt;
+    const double Ji = 1.0 / J;
+    m[0] = (y_et * z_zt - y_zt * z_et) * Ji;
+    m[1] = (x_zt * z_et - x_et * z_zt) * Ji;
+    m[2] = (x_et * y_zt - x_zt * y_et) * Ji;
+    m[3] = (y_zt * z_xi - y_xi * z_zt) * Ji;
+    m[4] = (x_xi * z_zt - x_zt * z_xi) * Ji;
+    m[5] = (x_zt * y_xi - x_xi * y_zt) * Ji;
+    m[6] = (y_xi * z_et - y_et * z_xi) * Ji;
+    m[7] = (x_et * z_xi - x_xi * z_et) * Ji;
+    m[8] = (x_xi * y_et - x_et * y_xi) * Ji;
+}
+
+// mu, lambda, buoyancy with the storage scaling folded in (reference calculate_medium,
+// CJM.cpp:46-73): rho' = rho*Cs/Cv, mu = rho' Vs^2, lambda = rho'(Vp^2 - 2Vs^2),
+// buoyancy = 1/(rho' Crho).
+__device__ __forceinline__ void medium_at(const SetupParams& q, float vs, float vp, float rho, double& mu,
+                                          double& lambda, double& buoy) {
+    double r = (double)rho * (1.0 * q.Cs / q.Cv);
+    double Vs = vs, Vp = vp;
+    mu = r * (Vs * Vs);
+    lambda = r * (Vp * Vp - 2.0 * Vs * Vs);
+    buoy = 1.0 / r / q.Crho;
+}
+
+template <class S>
+__global__ void k_medium(SetupParams q, const float* __restrict__ vs, const float* __restrict__ vp,
+                         const float* __restrict__ rho, S* __restrict__ cjm) {
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)q.nx * q.ny * q.nz;
+    if (t >= n) return;
+    int i = (int)(t % q.nx), j = (int)((t / q.nx) % q.ny), k = (int)(t / ((long long)q.nx * q.ny));
+    long long idx = (i + HALO) + (long long)(j + HALO) * q.nx_ + (long long)(k + HALO) * q.nx_ * q.ny_;
+    double mu, lambda, buoy;
+    medium_at(q, vs[t], vp[t], rho[t], mu, lambda, buoy);
+    cjm[10 * q.num + idx] = SV<S>::st((float)mu);
+    cjm[11 * q.num + idx] = SV<S>::st((float)lambda);
+    cjm[12 * q.num + idx] = SV<S>::st((float)buoy);
+}
+
+// Metric and J on the interior and, with a free surface, the mirrored values above it
+// (solve_coordinate_on_free_surface, contravariant.cpp:393-408: value at 2(_nz-1)-k).
+template <class S>
+__global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
+    const int _nz = q.nz_ - HALO;
+    const int kTop = q.fs ? q.nz_ : _nz;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)q.nx * q.ny * (kTop - HALO);
+    if (t >= n) return;
+    int i = (int)(t % q.nx) + HALO, j = (int)((t / q.nx) % q.ny) + HALO;
+    int k = (int)(t / ((long long)q.nx * q.ny)) + HALO;
+    int kk = k < _nz ? k : 2 * (_nz - 1) - k;
+    double m[9], J;
+    metric_at(q, i, j, kk, m, J);
+    long long idx = i + (long long)j * q.nx_ + (long long)k * q.nx_ * q.ny_;
+    for (int c = 0; c < 9; ++c) cjm[c * q.num + idx] = SV<S>::st((float)m[c]);
+    cjm[9 * q.num + idx] = SV<S>::st((float)J);
+}
+
+// Free-surface matrices -rDZ^-1 DX and -rDZ^-1 DY at the surface layer
+// (contravariant.cpp:257-351, including the reference's chi = 2*lambda + mu).
+__global__ void k_rdz(SetupParams q, const float* __restrict__ vs, const float* __restrict__ vp,
+                      const float* __restrict__ rho, float* __restrict__ rdz) {
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)q.nx * q.ny) return;
+    int i0 = (int)(t % q.nx), j0 = (int)(t / q.nx);
+    int i = i0 + HALO, j = j0 + HALO, k = q.nz_ - HALO - 1;
+    long long p = i0 + (long long)j0 * q.nx + (long long)(q.nz - 1) * q.nx * q.ny;
+    double m[9], J, mu, lambda, buoy;
+    metric_at(q, i, j, k, m, J);
+    medium_at(q, vs[p], vp[p], rho[p], mu, lambda, buoy);
+    const double xi_x = m[0], xi_y = m[1], xi_z = m[2], et_x = m[3], et_y = m[4], et_z = m[5];
+    const double zt_x = m[6], zt_y = m[7], zt_z = m[8];
+    const double chi = 2.0 * lambda + mu, lam_mu = lambda + mu;
+    double DZ[9], DX[9], DY[9], rD[9];
+    DZ[0] = chi * zt_x * zt_x + mu * (zt_y * zt_y + zt_z * zt_z);
+    DZ[1] = lam_mu * zt_x * zt_y;
+    DZ[2] = lam_mu * zt_x * zt_z;
+    DZ[3] = DZ[1];
+    DZ[4] = chi * zt_y * zt_y + mu * (zt_x * zt_x + zt_z * zt_z);
+    DZ[5] = lam_mu * zt_y * zt_z;
+    DZ[6] = DZ[2];
+    DZ[7] = DZ[5];
+    DZ[8] = chi * zt_z * zt_z + mu * (zt_x * zt_x + zt_y * zt_y);
+    const double det = DZ[0] * DZ[4] * DZ[8] + DZ[1] * DZ[5] * DZ[6] + DZ[2] * DZ[7] * DZ[3] - DZ[2] * DZ[4] * DZ[6] -
+                       DZ[1] * DZ[3] * DZ[8] - DZ[0] * DZ[7] * DZ[5];
+    DX[0] = chi * zt_x * xi_x + mu * (zt_y * xi_y + zt_z * xi_z);
+    DX[1] = lambda * zt_x * xi_y + mu * zt_y * xi_x;
+    DX[2] = lambda * zt_x * xi_z + mu * zt_z * xi_x;
+    DX[3] = lambda * zt_y * xi_x + mu * zt_x * xi_y;
+    DX[4] = chi * zt_y * xi_y + mu * (zt_x * xi_x + zt_z * xi_z);
+    DX[5] = lambda * zt_y * xi_z + mu * zt_z * xi_y;
+    DX[6] = lambda * zt_z * xi_x + mu * zt_x * xi_z;
+    DX[7] = lambda * zt_z * xi_y + mu * zt_y * xi_z;
+    DX[8] = chi * zt_z * xi_z + mu * (zt_x * xi_x + zt_y * xi_y);
+    DY[0] = chi * zt_x * et_x + mu * (zt_y * et_y + zt_z * et_z);
+    DY[1] = lambda * zt_x * et_y + mu * zt_y * et_x;
+    DY[2] = lambda * zt_x * et_z + mu * zt_z * et_x;
+    DY[3] = lambda * zt_y * et_x + mu * zt_x * et_y;
+    DY[4] = chi * zt_y * et_y + mu * (zt_x * et_x + zt_z * et_z);
+    DY[5] = lambda * zt_y * et_z + mu * zt_z * et_y;
+    DY[6] = lambda * zt_z * et_x + mu * zt_x * et_z;
+    DY[7] = lambda * zt_z * et_y + mu * zt_y * et_z;
+    DY[8] = chi * zt_z * et_z + mu * (zt_x * et_x + zt_y * et_y);
+    rD[0] = (DZ[4] * DZ[8] - DZ[5] * DZ[7]) / det;
+    rD[1] = (-DZ[3] * DZ[8] + DZ[5] * DZ[6]) / det;
+    rD[2] = (DZ[3] * DZ[7] - DZ[4] * DZ[6]) / det;
+    rD[3] = (-DZ[1] * DZ[8] + DZ[2] * DZ[7]) / det;
+    rD[4] = (DZ[0] * DZ[8] - DZ[2] * DZ[6]) / det;
+    rD[5] = (-DZ[0] * DZ[7] + DZ[1] * DZ[6]) / det;
+    rD[6] = (DZ[1] * DZ[5] - DZ[2] * DZ[4]) / det;
+    rD[7] = (-DZ[0] * DZ[5] + DZ[2] * DZ[3]) / det;
+    rD[8] = (DZ[0] * DZ[4] - DZ[1] * DZ[3]) / det;
+    const long long nxy = (long long)q.nx_ * q.ny_;
+    const long long s = (i + (long long)j * q.nx_) * 9;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            rdz[s + r * 3 + c] = (float)(-(rD[r * 3 + 0] * DX[c] + rD[r * 3 + 1] * DX[3 + c] + rD[r * 3 + 2] * DX[6 + c]));
+            rdz[nxy * 9 + s + r * 3 + c] =
+                (float)(-(rD[r * 3 + 0] * DY[c] + rD[r * 3 + 1] * DY[3 + c] + rD[r * 3 + 2] * DY[6 + c]));
+        }
+}
+
+// Smallest point-to-plane distance among the 8 neighbour triangles of every interior point
+// (reference calculate_DH_Range / distance_point2plane, calc_CFL.cpp:98-170), reduced with an
+// atomic minimum on the bit pattern of a positive double.
+__global__ void k_hmin(SetupParams q, unsigned long long* out) {
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)q.nx * q.ny * q.nz;
+    double hmin = 1.0e20;
+    if (t < n) {
+        int i = (int)(t % q.nx) + HALO, j = (int)((t / q.nx) % q.ny) + HALO;
+        int k = (int)(t / ((long long)q.nx * q.ny)) + HALO;
+        double P[3], A[3], B[3], C[3];
+        coord_at(q, i, j, k, P);
+        for (int ii = -1; ii <= 1; ii += 2)
+            for (int jj = -1; jj <= 1; jj += 2)
+                for (int kk = -1; kk <= 1; kk += 2) {
+                    coord_at(q, i - ii, j, k, A);
+                    coord_at(q, i, j - jj, k, B);
+                    coord_at(q, i, j, k - kk, C);
+                    double AB[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+                    double AC[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+                    double nn[3] = {AB[1] * AC[2] - AB[2] * AC[1], AB[2] * AC[0] - AB[0] * AC[2],
+                                    AB[0] * AC[1] - AB[1] * AC[0]};
+                    double nd = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                    double d = fabs(((P[0] - A[0]) * nn[0] + (P[1] - A[1]) * nn[1] + (P[2] - A[2]) * nn[2]) / nd);
+                    hmin = fmin(hmin, d);
+                }
+    }
+    for (int off = 32; off > 0; off >>= 1) hmin = fmin(hmin, __shfl_down(hmin, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMin(out, (unsigned long long)__double_as_longlong(hmin));
+}
+
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int launch_setup_medium(int prec, const SetupParams& q, const float* vs, const float* vp, const float* rho, void* cjm,
+                        hipStream_t s) {
+    long long n = (long long)q.nx * q.ny * q.nz;
+    const int bs = 256;
+    switch (prec) {
+        case P_F32: k_medium<float><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (float*)cjm); break;
+        case P_F16: k_medium<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (_Float16*)cjm); break;
+        case P_BF16: k_medium<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (__bf16*)cjm); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s) {
+    const int kTop = q.fs ? q.nz_ : q.nz_ - HALO;
+    long long n = (long long)q.nx * q.ny * (kTop - HALO);
+    const int bs = 256;
+    switch (prec) {
+        case P_F32: k_metric<float><<<blocks_for(n, bs), bs, 0, s>>>(q, (float*)cjm); break;
+        case P_F16: k_metric<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, (_Float16*)cjm); break;
+        case P_BF16: k_metric<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, (__bf16*)cjm); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_setup_rdz(const SetupParams& q, const float* vs, const float* vp, const float* rho, float* rdz,
+                     hipStream_t s) {
+    long long n = (long long)q.nx * q.ny;
+    k_rdz<<<blocks_for(n, 256), 256, 0, s>>>(q, vs, vp, rho, rdz);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s) {
+    long long n = (long long)q.nx * q.ny * q.nz;
+    k_hmin<<<blocks_for(n, 256), 256, 0, s>>>(q, out);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace cg

@@ -1,0 +1,295 @@
+// stage_general.hpp — one RK stage at one grid point, every boundary branch included.
+//
+// Per point this reproduces the reference's per-stage composite, in the reference order:
+//   wave_deriv (wave_deriv.cpp:58-203)            -> h
+//   free_surface_deriv on the top 3 layers        -> h overwritten (freeSurface.cpp:22-258)
+//   pml_deriv x1,y1,z1,x2,y2[,z2]                 -> h += -beta*Aux*DT, hAux (pml_deriv.cpp:90-544)
+//   pml_free_surface x1,y1,x2,y2 at the surface   -> hAux stress += ... (pml_freeSurface.cpp:12-235)
+//   wave_rk / wave_pml_rk                         -> W, Aux (wave_rk.cpp:13-99, pml_rk.cpp:14-70)
+// The stage increment h stays in FP32 registers instead of a 9-plane h_W array, and the RK
+// update writes the next stage's field into a separate buffer (ping-pong), so the derivative
+// and the update fuse into one pass without a read-after-write hazard.
+#pragma once
+#include "common.hpp"
+
+namespace cg {
+
+template <class S>
+__device__ __forceinline__ float Lraw(const S* __restrict__ w, long long idx, long long st, int FB, float rDH) {
+    // FB*(a_-1 f[-FB] + a0 f[0] + a1 f[FB] + a2 f[2FB] + a3 f[3FB])*rDH  (axis.h:129-133)
+    const long long s = FB * st;
+    return (float)FB *
+           (af_1 * SV<S>::ld(w[idx - s]) + af0 * SV<S>::ld(w[idx]) + af1 * SV<S>::ld(w[idx + s]) +
+            af2 * SV<S>::ld(w[idx + 2 * s]) + af3 * SV<S>::ld(w[idx + 3 * s])) *
+           rDH;
+}
+
+__device__ __forceinline__ float LJT(const float* J_T, int FB, float rDH) {
+    return (float)FB *
+           (af_1 * J_T[3 - FB] + af0 * J_T[3] + af1 * J_T[3 + FB] + af2 * J_T[3 + 2 * FB] + af3 * J_T[3 + 3 * FB]) * rDH;
+}
+
+// RK register update of the 9 wavefield variables at idx (wave_rk0..3).
+template <class S>
+__device__ __forceinline__ void rk_update(const StageParams& p, long long idx, const float* h) {
+    S* __restrict__ Wo = (S*)p.Wout;
+    const S* __restrict__ A = (const S*)p.A;
+    S* __restrict__ T = (S*)p.T;
+    const long long num = p.num;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        const long long q = v * num + idx;
+        if (p.irk == 0) {
+            float m = SV<S>::ld(A[q]);
+            Wo[q] = SV<S>::st(m + alpha2 * h[v]);
+            T[q] = SV<S>::st(m + beta1 * h[v]);
+        } else if (p.irk == 1) {
+            float m = SV<S>::ld(A[q]), t = SV<S>::ld(T[q]);
+            Wo[q] = SV<S>::st(m + alpha3 * h[v]);
+            T[q] = SV<S>::st(t + beta2 * h[v]);
+        } else if (p.irk == 2) {
+            float m = SV<S>::ld(A[q]), t = SV<S>::ld(T[q]);
+            Wo[q] = SV<S>::st(m + h[v]);
+            T[q] = SV<S>::st(t + beta3 * h[v]);
+        } else {
+            float t = SV<S>::ld(T[q]);
+            Wo[q] = SV<S>::st(t + beta4 * h[v]);
+        }
+    }
+}
+
+template <class S>
+__device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
+    const S* __restrict__ W = (const S*)p.Win;
+    const S* __restrict__ Cj = (const S*)p.cjm;
+    const long long num = p.num;
+    const long long sx = 1, sy = p.nx_, sz = (long long)p.nx_ * p.ny_;
+    const long long idx = i + j * sy + k * sz;
+    const float rDH = p.rDH, DT = p.DT;
+    const int _nz = p.nz + HALO;
+    const int FBs[3] = {p.FB1, p.FB2, p.FB3};
+    const long long strd[3] = {sx, sy, sz};
+
+    float M[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) M[c] = SV<S>::ld(Cj[c * num + idx]);
+    const float mu = SV<S>::ld(Cj[10 * num + idx]);
+    const float lambda = SV<S>::ld(Cj[11 * num + idx]);
+    const float buoyancy = SV<S>::ld(Cj[12 * num + idx]) * p.Crho;
+
+    // raw one-sided derivatives D[dir][var] (no PML factor)
+    float D[3][9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < 9; ++v) D[d][v] = Lraw<S>(W + v * num, idx, strd[d], FBs[d], rDH);
+
+    const float bx = p.pb[0][i], by = p.pb[1][j], bz = p.pb[2][k];
+    float h[9];
+    const bool fsl = p.fs && k >= _nz - HALO;
+    if (!fsl) {
+        // wave_deriv.cpp:132-201
+        float B[3][9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            B[0][v] = D[0][v] * bx;
+            B[1][v] = D[1][v] * by;
+            B[2][v] = D[2][v] * bz;
+        }
+        float hv[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) hv[v] = 0.f;
+        float Vx[3], Vy[3], Vz[3], Txx[3], Tyy[3], Tzz[3], Txy[3], Txz[3], Tyz[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float mx = M[3 * d], my = M[3 * d + 1], mz = M[3 * d + 2];
+            const float* b = B[d];
+            Vx[d] = (mx * b[3] + my * b[6] + mz * b[7]) * buoyancy;
+            Vy[d] = (mx * b[6] + my * b[4] + mz * b[8]) * buoyancy;
+            Vz[d] = (mx * b[7] + my * b[8] + mz * b[5]) * buoyancy;
+            const float dv = (mx * b[0] + my * b[1] + mz * b[2]) * lambda;
+            Txx[d] = dv + 2.0f * mu * (mx * b[0]);
+            Tyy[d] = dv + 2.0f * mu * (my * b[1]);
+            Tzz[d] = dv + 2.0f * mu * (mz * b[2]);
+            Txy[d] = (my * b[0] + mx * b[1]) * mu;
+            Txz[d] = (mz * b[0] + mx * b[2]) * mu;
+            Tyz[d] = (mz * b[1] + my * b[2]) * mu;
+        }
+        h[0] = (Vx[0] + Vx[1] + Vx[2]) * DT;
+        h[1] = (Vy[0] + Vy[1] + Vy[2]) * DT;
+        h[2] = (Vz[0] + Vz[1] + Vz[2]) * DT;
+        h[3] = (Txx[0] + Txx[1] + Txx[2]) * DT;
+        h[4] = (Tyy[0] + Tyy[1] + Tyy[2]) * DT;
+        h[5] = (Tzz[0] + Tzz[1] + Tzz[2]) * DT;
+        h[6] = (Txy[0] + Txy[1] + Txy[2]) * DT;
+        h[7] = (Txz[0] + Txz[1] + Txz[2]) * DT;
+        h[8] = (Tyz[0] + Tyz[1] + Tyz[2]) * DT;
+        (void)hv;
+    } else {
+        // freeSurface.cpp:104-256: conservative J*(metric . T) form for the velocity with
+        // antisymmetric imaging of the stress about the surface
+        const int k_s = (_nz - 1) - k + HALO;
+        float JT[3][3][7];  // [dir][component][l]
+#pragma unroll
+        for (int l = 0; l <= 2 * HALO; ++l) {
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const long long pos = idx + (l - HALO) * strd[d];
+                const float m0 = SV<S>::ld(Cj[(3 * d) * num + pos]);
+                const float m1 = SV<S>::ld(Cj[(3 * d + 1) * num + pos]);
+                const float m2 = SV<S>::ld(Cj[(3 * d + 2) * num + pos]);
+                const float jac = SV<S>::ld(Cj[9 * num + pos]);
+                const float w3 = SV<S>::ld(W[3 * num + pos]), w4 = SV<S>::ld(W[4 * num + pos]);
+                const float w5 = SV<S>::ld(W[5 * num + pos]), w6 = SV<S>::ld(W[6 * num + pos]);
+                const float w7 = SV<S>::ld(W[7 * num + pos]), w8 = SV<S>::ld(W[8 * num + pos]);
+                JT[d][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
+                JT[d][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
+                JT[d][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
+            }
+        }
+        for (int l = 0; l < k_s; ++l) {
+            const long long pos = idx + (l - HALO) * sz;
+            const float m0 = SV<S>::ld(Cj[6 * num + pos]);
+            const float m1 = SV<S>::ld(Cj[7 * num + pos]);
+            const float m2 = SV<S>::ld(Cj[8 * num + pos]);
+            const float jac = SV<S>::ld(Cj[9 * num + pos]);
+            const float w3 = SV<S>::ld(W[3 * num + pos]), w4 = SV<S>::ld(W[4 * num + pos]);
+            const float w5 = SV<S>::ld(W[5 * num + pos]), w6 = SV<S>::ld(W[6 * num + pos]);
+            const float w7 = SV<S>::ld(W[7 * num + pos]), w8 = SV<S>::ld(W[8 * num + pos]);
+            JT[2][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
+            JT[2][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
+            JT[2][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
+        }
+        for (int c = 0; c < 3; ++c) {
+            JT[2][c][k_s] = 0.f;
+            for (int l = k_s + 1; l <= 2 * HALO; ++l) JT[2][c][l] = -JT[2][c][2 * k_s - l];
+        }
+        const float jacb = SV<S>::ld(Cj[9 * num + idx]);
+        const float Jinv = 1.0f / jacb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            h[c] = buoyancy * Jinv *
+                   (LJT(JT[0][c], p.FB1, rDH) * bx + LJT(JT[1][c], p.FB2, rDH) * by + LJT(JT[2][c], p.FB3, rDH)) * DT;
+
+        const float Vx_xi = D[0][0] * bx, Vx_et = D[1][0] * by;
+        const float Vy_xi = D[0][1] * bx, Vy_et = D[1][1] * by;
+        const float Vz_xi = D[0][2] * bx, Vz_et = D[1][2] * by;
+        float Vx_zt, Vy_zt, Vz_zt;
+        if (k == _nz - 1) {
+            const float* X = p.rdz + (i + j * sy) * 9;
+            const float* Y = p.rdz + (long long)p.nx_ * p.ny_ * 9 + (i + j * sy) * 9;
+            Vx_zt = (X[0] * Vx_xi + X[1] * Vy_xi + X[2] * Vz_xi) + (Y[0] * Vx_et + Y[1] * Vy_et + Y[2] * Vz_et);
+            Vy_zt = (X[3] * Vx_xi + X[4] * Vy_xi + X[5] * Vz_xi) + (Y[3] * Vx_et + Y[4] * Vy_et + Y[5] * Vz_et);
+            Vz_zt = (X[6] * Vx_xi + X[7] * Vy_xi + X[8] * Vz_xi) + (Y[6] * Vx_et + Y[7] * Vy_et + Y[8] * Vz_et);
+        } else if (k == _nz - 2) {  // L2 (axis.h:92)
+            const long long s = p.FB3 * sz;
+            Vx_zt = (float)p.FB3 * (SV<S>::ld(W[0 * num + idx + s]) - SV<S>::ld(W[0 * num + idx])) * rDH;
+            Vy_zt = (float)p.FB3 * (SV<S>::ld(W[1 * num + idx + s]) - SV<S>::ld(W[1 * num + idx])) * rDH;
+            Vz_zt = (float)p.FB3 * (SV<S>::ld(W[2 * num + idx + s]) - SV<S>::ld(W[2 * num + idx])) * rDH;
+        } else {  // L3 (axis.h:93)
+            const long long s = p.FB3 * sz;
+            float r3[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v)
+                r3[v] = (float)p.FB3 *
+                        (Cf1 * SV<S>::ld(W[v * num + idx]) + Cf2 * SV<S>::ld(W[v * num + idx + s]) +
+                         Cf3 * SV<S>::ld(W[v * num + idx + 2 * s])) *
+                        rDH;
+            Vx_zt = r3[0]; Vy_zt = r3[1]; Vz_zt = r3[2];
+        }
+        const float Vd[3][3] = {{Vx_xi, Vy_xi, Vz_xi}, {Vx_et, Vy_et, Vz_et}, {Vx_zt, Vy_zt, Vz_zt}};
+        float Txx = 0.f, Tyy = 0.f, Tzz = 0.f, Txy = 0.f, Txz = 0.f, Tyz = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float mx = M[3 * d], my = M[3 * d + 1], mz = M[3 * d + 2];
+            const float* b = Vd[d];
+            const float dv = (mx * b[0] + my * b[1] + mz * b[2]) * lambda;
+            Txx += dv + 2.0f * mu * (mx * b[0]);
+            Tyy += dv + 2.0f * mu * (my * b[1]);
+            Tzz += dv + 2.0f * mu * (mz * b[2]);
+            Txy += (my * b[0] + mx * b[1]) * mu;
+            Txz += (mz * b[0] + mx * b[2]) * mu;
+            Tyz += (mz * b[1] + my * b[2]) * mu;
+        }
+        h[3] = Txx * DT; h[4] = Tyy * DT; h[5] = Tzz * DT;
+        h[6] = Txy * DT; h[7] = Txz * DT; h[8] = Tyz * DT;
+    }
+
+    if (p.pml) {
+        const int ijk[3] = {i, j, k};
+#pragma unroll 1
+        for (int s = 0; s < 6; ++s) {
+            const Slab& X = p.slab[s];
+            if (!X.on) continue;
+            const int a = X.axis;
+            const int c = ijk[a];
+            if (c < X.lo || c >= X.lo + p.nPML) continue;
+            const int i0 = (a == 0 ? c - X.lo : i - HALO);
+            const int j0 = (a == 1 ? c - X.lo : j - HALO);
+            const int k0 = (a == 2 ? c - X.lo : k - HALO);
+            const long long pos = i0 + (long long)j0 * X.n0 + (long long)k0 * X.n0 * X.n1;
+            const float beta = p.pb[a][c], dd = p.pd[a][c], alpha_d = dd + p.pa[a][c];
+            float aold[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) aold[v] = X.a[v * X.n + pos];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) h[v] += -beta * aold[v] * DT;
+            // single-direction right-hand side with d (pml_deriv.cpp:176-207)
+            const float mx = M[3 * a], my = M[3 * a + 1], mz = M[3 * a + 2];
+            float b[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) b[v] = D[a][v] * dd;
+            float r[9];
+            r[0] = (mx * b[3] + my * b[6] + mz * b[7]) * buoyancy;
+            r[1] = (mx * b[6] + my * b[4] + mz * b[8]) * buoyancy;
+            r[2] = (mx * b[7] + my * b[8] + mz * b[5]) * buoyancy;
+            const float dv = (mx * b[0] + my * b[1] + mz * b[2]) * lambda;
+            r[3] = dv + 2.0f * mu * (mx * b[0]);
+            r[4] = dv + 2.0f * mu * (my * b[1]);
+            r[5] = dv + 2.0f * mu * (mz * b[2]);
+            r[6] = (my * b[0] + mx * b[1]) * mu;
+            r[7] = (mz * b[0] + mx * b[2]) * mu;
+            r[8] = (mz * b[1] + my * b[2]) * mu;
+            float ha[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) ha[v] = (r[v] - alpha_d * aold[v]) * DT;
+            if (p.fs && a < 2 && k == _nz - 1) {
+                // pml_free_surface_x/y (pml_freeSurface.cpp:81-116)
+                const float* Mz = p.rdz + (a == 0 ? 0 : (long long)p.nx_ * p.ny_ * 9) + (i + j * sy) * 9;
+                const float zx = M[6], zy = M[7], zz = M[8];
+                const float Vz0 = Mz[0] * b[0] + Mz[1] * b[1] + Mz[2] * b[2];
+                const float Vz1 = Mz[3] * b[0] + Mz[4] * b[1] + Mz[5] * b[2];
+                const float Vz2 = Mz[6] * b[0] + Mz[7] * b[1] + Mz[8] * b[2];
+                const float dz = (zx * Vz0 + zy * Vz1 + zz * Vz2) * lambda;
+                ha[3] += (dz + 2.0f * mu * (zx * Vz0)) * DT;
+                ha[4] += (dz + 2.0f * mu * (zy * Vz1)) * DT;
+                ha[5] += (dz + 2.0f * mu * (zz * Vz2)) * DT;
+                ha[6] += ((zy * Vz0 + zx * Vz1) * mu) * DT;
+                ha[7] += ((zz * Vz0 + zx * Vz2) * mu) * DT;
+                ha[8] += ((zz * Vz1 + zy * Vz2) * mu) * DT;
+            }
+            // wave_pml_rk0..3 (pml_rk.cpp:14-70), in place: each point owns its aux values
+#pragma unroll
+            for (int v = 0; v < 9; ++v) {
+                const long long q = v * X.n + pos;
+                if (p.irk == 0) {
+                    const float m = aold[v];
+                    X.m[q] = m;
+                    X.t[q] = m + beta1 * ha[v];
+                    X.a[q] = m + alpha2 * ha[v];
+                } else if (p.irk == 1) {
+                    X.t[q] = X.t[q] + beta2 * ha[v];
+                    X.a[q] = X.m[q] + alpha3 * ha[v];
+                } else if (p.irk == 2) {
+                    X.t[q] = X.t[q] + beta3 * ha[v];
+                    X.a[q] = X.m[q] + ha[v];
+                } else {
+                    X.a[q] = X.t[q] + beta4 * ha[v];
+                }
+            }
+        }
+    }
+    rk_update<S>(p, idx, h);
+}
+
+}  // namespace cg

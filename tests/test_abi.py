@@ -1,0 +1,85 @@
+"""The C-ABI library: it loads, exports every entry point include/cgfdm.h declares, its descriptor
+layout matches the header as a C compiler sees it, and argument errors are reported before any
+device call (these run without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cgfdm.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cgfdm_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(gpu):
+    lib = gpu.lib()
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_desc_layout_matches_header(gpu, tmp_path):
+    fields = [f[0] for f in gpu.Desc._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){",
+             'printf("size %zu\\n", sizeof(cgfdm_desc));']
+    lines += [f'printf("{f} %zu\\n", offsetof(cgfdm_desc, {f}));' for f in fields]
+    lines += ["return 0;}"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-o", str(exe), str(src)])
+    out = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    assert int(out["size"]) == ctypes.sizeof(gpu.Desc)
+    for f in fields:
+        assert int(out[f]) == getattr(gpu.Desc, f).offset, f
+
+
+def _create_rc(gpu, desc):
+    h = ctypes.c_void_p()
+    rc = gpu.lib().cgfdm_domain_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    return rc, gpu.lib().cgfdm_last_error().decode()
+
+
+def test_pml_wider_than_grid_is_rejected(gpu):
+    d = gpu.make_desc(10, 40, 40, nPML=12)
+    rc, msg = _create_rc(gpu, d)
+    assert rc == -5 and "nPML" in msg
+
+
+def test_bad_descriptors_are_rejected(gpu):
+    d = gpu.make_desc(20, 20, 20, nPML=6)
+    d.precision = 7
+    assert _create_rc(gpu, d)[0] == -1
+    d = gpu.make_desc(20, 20, 20, nPML=6, DT=0.0)
+    assert _create_rc(gpu, d)[0] == -1
+    d = gpu.make_desc(2000, 2000, 600, nPML=12)  # > 2^31 padded points
+    rc, msg = _create_rc(gpu, d)
+    assert rc == -1 and "2^31" in msg
+    h = ctypes.c_void_p()
+    assert gpu.lib().cgfdm_domain_create(None, 0, ctypes.byref(h)) == -1
+
+
+def test_estimate_bytes_scales_with_precision(gpu):
+    d32 = gpu.make_desc(201, 201, 101, precision="fp32")
+    d16 = gpu.make_desc(201, 201, 101, precision="fp16")
+    b32, b16 = gpu.estimate_bytes(d32), gpu.estimate_bytes(d16)
+    num = 207 * 207 * 107
+    assert b32 >= num * 4 * 49 and b16 >= num * 2 * 49
+    assert 1.6 < b32 / b16 < 2.0  # PML auxiliaries stay FP32
+
+
+def test_no_fallback_when_library_missing(gpu, monkeypatch):
+    import importlib
+    import cgfdm
+    monkeypatch.setattr(cgfdm, "LIB_PATH", "/nonexistent/libcgfdm.so")
+    monkeypatch.setattr(cgfdm, "_lib", None)
+    with pytest.raises(RuntimeError):
+        cgfdm.lib()
