@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcgfdm.so")
+CLI = os.path.join(HERE, "cgfdm_run")
 SOURCES = ["setup.hip", "stage.hip", "misc.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,6 +48,13 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # command-line driver: plain C++ over the C ABI
+    cmd = ["g++", "-O2", "-std=c++17", "-o", CLI + ".tmp", os.path.join(CSRC, "cgfdm_run.cpp"),
+           "-L" + HERE, "-lcgfdm", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cgfdm_run build failed:\n" + r.stdout + r.stderr)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcgfdm.so")
+LIB_PATH = os.environ.get("CGFDM_LIB") or os.path.join(HERE, "libcgfdm.so")
 
 FP32, FP16, BF16 = 1, 2, 3
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}

@@ -1,14 +1,15 @@
 // api.hip — host implementation of the C ABI declared in include/cgfdm.h.
 //
 // One domain = one simulation on one GPU with its own HIP stream (default priority).
-// Device memory layout (padded grid, HALO = 3, x fastest; var stride num = _nx_*_ny_*_nz_):
-//   W buffers A, B, C and accumulator T : 9 x num storage values each
-//   CJM                                  : 13 x num storage values
+// Device memory layout: every 3-D array is k-major, [k][var][j][i] (see Layout in common.hpp):
+//   W buffers A, B, C and accumulator T : 9 variables, storage type
+//   CJM                                  : 13 components, storage type
 //   PML slabs                            : 3 registers x 9 x slab points, FP32
 //   rdz, profiles, stations, PGV         : FP32
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -62,7 +63,7 @@ struct cgfdm_domain {
     int prec = P_F32;
     int sb = 4;  // storage bytes
     int nx = 0, ny = 0, nz = 0, nx_ = 0, ny_ = 0, nz_ = 0;
-    long long num = 0;
+    Layout lay{};
     double Cv = 1, Cs = 1, Crho = 1;
     void* W[4] = {nullptr, nullptr, nullptr, nullptr};  // A, B, C, T
     void* cjm = nullptr;
@@ -71,8 +72,8 @@ struct cgfdm_domain {
     float* auxmem = nullptr;
     Slab slab[6]{};
     float* pgv = nullptr;
-    // stations
-    long long* stIdx = nullptr;
+    // stations (padded i, j, k triples)
+    int* stIdx = nullptr;
     float* stWave = nullptr;
     int nst = 0;
     // sources
@@ -80,7 +81,7 @@ struct cgfdm_domain {
     int SX = 0, SY = 0, SZ = 0;
     float fc = 0.f;
     float* srcCoef = nullptr;
-    long long* srcIdx = nullptr;
+    int* srcIdx = nullptr;  // padded i, j, k triples
     float* mrate = nullptr;
     long long npts = 0;
     int snt = 0;
@@ -88,6 +89,7 @@ struct cgfdm_domain {
     bool multiActive = false;
     int it = 0;
     bool modelSet = false;
+    int bulk_cfg = 0, bulk_kc = 0;  // launch shape of the bulk stage kernel
     long long bytes = 0;
     // timing
     bool timing = false;
@@ -153,10 +155,23 @@ int validate(const cgfdm_desc* d) {
     return 0;
 }
 
+// Internal layout: rows padded to a multiple of 32 elements and shifted so that the first column of
+// the bulk box (3 + nPML) is 64-byte aligned for both storage widths.
+Layout make_layout(const cgfdm_desc& d) {
+    Layout L{};
+    L.nx_ = d.NX + 6; L.ny_ = d.NY + 6; L.nz_ = d.NZ + 6;
+    const int align = 32;
+    const int bi0 = HALO + (d.pml ? d.nPML : 0);
+    L.lead = (align - bi0 % align) % align;
+    L.px = (L.lead + L.nx_ + align - 1) / align * align;
+    L.plane = (long long)L.px * L.ny_;
+    return L;
+}
+
 long long estimate(const cgfdm_desc& d) {
-    long long num = (long long)(d.NX + 6) * (d.NY + 6) * (d.NZ + 6);
+    const Layout L = make_layout(d);
     int sb = d.precision == CGFDM_FP32 ? 4 : 2;
-    long long b = num * sb * (9LL * 4 + 13);
+    long long b = (L.size(9) * 4 + L.size(13)) * sb;
     for (int s = 0; s < 6; ++s) b += slab_points(d, s) * 9 * 3 * 4;
     b += (long long)(d.NX + 6) * (d.NY + 6) * 18 * 4;
     b += 3LL * (d.NX + d.NY + d.NZ + 18) * 4;
@@ -202,7 +217,7 @@ SetupParams setup_params(const cgfdm_domain* h, const double* dterrain) {
     SetupParams q{};
     q.nx = h->nx; q.ny = h->ny; q.nz = h->nz;
     q.nx_ = h->nx_; q.ny_ = h->ny_; q.nz_ = h->nz_;
-    q.num = h->num;
+    q.lay = h->lay;
     q.DH = h->d.DH;
     q.use_terrain = h->d.use_terrain;
     q.depth = h->d.depth;
@@ -254,17 +269,19 @@ int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) 
     h->sb = storage_bytes(h->prec);
     h->nx = desc->NX; h->ny = desc->NY; h->nz = desc->NZ;
     h->nx_ = h->nx + 6; h->ny_ = h->ny + 6; h->nz_ = h->nz + 6;
-    h->num = (long long)h->nx_ * h->ny_ * h->nz_;
+    h->lay = make_layout(*desc);
     const bool low = h->prec != P_F32;
     h->Cv = desc->Cv > 0 ? desc->Cv : (low ? 5.0e3 : 1.0);
     h->Cs = desc->Cs > 0 ? desc->Cs : (low ? 1.0e-6 : 1.0);
     h->Crho = desc->Crho > 0 ? desc->Crho : (low ? 1.0e6 : 1.0);
+    if (const char* e = std::getenv("CGFDM_BULK_CFG")) h->bulk_cfg = std::atoi(e);
+    if (const char* e = std::getenv("CGFDM_BULK_KC")) h->bulk_kc = std::atoi(e);
     hipError_t e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) { set_err("hipStreamCreate: %s", hipGetErrorString(e)); delete h; return CGFDM_EHIP; }
     auto fail = [&](int code) { cgfdm_domain_destroy(h); return code; };
     for (int b = 0; b < 4; ++b)
-        if ((rc = h->alloc(&h->W[b], (size_t)h->num * 9 * h->sb))) return fail(rc);
-    if ((rc = h->alloc(&h->cjm, (size_t)h->num * 13 * h->sb))) return fail(rc);
+        if ((rc = h->alloc(&h->W[b], (size_t)h->lay.size(9) * h->sb))) return fail(rc);
+    if ((rc = h->alloc(&h->cjm, (size_t)h->lay.size(13) * h->sb))) return fail(rc);
     // PML slabs: x1 y1 z1 x2 y2 z2
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
@@ -397,8 +414,8 @@ int cgfdm_set_source_point(cgfdm_domain* h, int X, int Y, int Z, float fc) {
     int rc = h->alloc((void**)&tmp, sizeof(float));
     if (rc) return rc;
     h->bytes -= 4;
-    const long long idx = SX + (long long)SY * h->nx_ + (long long)SZ * h->nx_ * h->ny_;
-    LCHK(launch_decode(h->prec, (const char*)h->cjm + (9 * h->num + idx) * h->sb, tmp, 1, h->stream));
+    const long long idx = h->lay.at(13, 9, SX, SY, SZ);
+    LCHK(launch_decode(h->prec, (const char*)h->cjm + idx * h->sb, tmp, 1, h->stream));
     float jacb = 0;
     HIPCHK(hipMemcpyAsync(&jacb, tmp, 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -442,8 +459,13 @@ int cgfdm_set_source_table(cgfdm_domain* h, const int* xyz, const float* rate, l
     }
     long long u = 0;
     for (auto& kv : uniq) kv.second = u++;
-    std::vector<long long> sidx(u);
-    for (auto& kv : uniq) sidx[kv.second] = kv.first;
+    std::vector<int> sidx(3 * u);
+    for (auto& kv : uniq) {
+        const long long key = kv.first, q = kv.second;
+        sidx[3 * q] = (int)(key % h->nx_);
+        sidx[3 * q + 1] = (int)((key / h->nx_) % h->ny_);
+        sidx[3 * q + 2] = (int)(key / ((long long)h->nx_ * h->ny_));
+    }
     std::vector<float> mr((size_t)nt * u * 6, 0.f);
     for (long long p = 0; p < npts; ++p) {
         const long long q = uniq[idxOf[p]];
@@ -453,11 +475,11 @@ int cgfdm_set_source_table(cgfdm_domain* h, const int* xyz, const float* rate, l
     h->release(h->srcIdx); h->srcIdx = nullptr;
     h->release(h->mrate); h->mrate = nullptr;
     int rc;
-    if ((rc = h->alloc((void**)&h->srcIdx, sizeof(long long) * u))) return rc;
+    if ((rc = h->alloc((void**)&h->srcIdx, sizeof(int) * 3 * u))) return rc;
     if ((rc = h->alloc((void**)&h->mrate, sizeof(float) * mr.size()))) return rc;
-    HIPCHK(hipMemcpyAsync(h->srcIdx, sidx.data(), sizeof(long long) * u, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->srcIdx, sidx.data(), sizeof(int) * 3 * u, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->mrate, mr.data(), sizeof(float) * mr.size(), hipMemcpyHostToDevice, h->stream));
-    LCHK(launch_scale_mrate(h->prec, h->cjm, h->num, h->srcIdx, h->mrate, u, nt, (float)h->d.DH, (float)h->Cv,
+    LCHK(launch_scale_mrate(h->prec, h->cjm, h->lay, h->srcIdx, h->mrate, u, nt, (float)h->d.DH, (float)h->Cv,
                             h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->npts = u; h->snt = nt; h->sdt = dt;
@@ -469,23 +491,23 @@ int cgfdm_set_source_table(cgfdm_domain* h, const int* xyz, const float* rate, l
 int cgfdm_set_stations(cgfdm_domain* h, const int* xyz, int n) {
     if (!h || (n > 0 && !xyz) || n < 0) { set_err("bad station list"); return CGFDM_EINVAL; }
     HIPCHK(hipSetDevice(h->dev));
-    std::vector<long long> idx(n);
+    std::vector<int> idx(3 * (size_t)n);
     for (int s = 0; s < n; ++s) {
         const int X = xyz[3 * s], Y = xyz[3 * s + 1], Z = xyz[3 * s + 2];
         if (X < 0 || X >= h->nx || Y < 0 || Y >= h->ny || Z < 0 || Z >= h->nz) {
             set_err("station %d (%d,%d,%d) outside the grid", s, X, Y, Z);
             return CGFDM_EINVAL;
         }
-        idx[s] = (X + HALO) + (long long)(Y + HALO) * h->nx_ + (long long)(Z + HALO) * h->nx_ * h->ny_;
+        idx[3 * s] = X + HALO; idx[3 * s + 1] = Y + HALO; idx[3 * s + 2] = Z + HALO;
     }
     h->release(h->stIdx); h->stIdx = nullptr;
     h->release(h->stWave); h->stWave = nullptr;
     h->nst = n;
     if (n == 0) return CGFDM_OK;
     int rc;
-    if ((rc = h->alloc((void**)&h->stIdx, sizeof(long long) * n))) return rc;
+    if ((rc = h->alloc((void**)&h->stIdx, sizeof(int) * 3 * n))) return rc;
     if ((rc = h->alloc((void**)&h->stWave, sizeof(float) * (size_t)n * h->d.NT * 9))) return rc;
-    HIPCHK(hipMemcpyAsync(h->stIdx, idx.data(), sizeof(long long) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->stIdx, idx.data(), sizeof(int) * 3 * n, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return CGFDM_OK;
 }
@@ -508,20 +530,22 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
     }
     p.rdz = h->rdz;
     for (int s = 0; s < 6; ++s) p.slab[s] = h->slab[s];
-    p.num = h->num;
+    p.lay = h->lay;
     p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
     p.nx_ = h->nx_; p.ny_ = h->ny_;
     p.nPML = h->d.nPML;
     p.fs = h->d.free_surface;
     p.pml = h->d.pml;
+    p.bulk_cfg = h->bulk_cfg;
+    p.bulk_kc = h->bulk_kc;
     p.rDH = (float)(1.0 / h->d.DH);
     p.DT = DT;
     p.Crho = (float)h->Crho;
     for (int n = 0; n < nsteps; ++n) {
         int FB1 = FB_TABLE[h->it % 8][0], FB2 = FB_TABLE[h->it % 8][1], FB3 = FB_TABLE[h->it % 8][2];
         if (h->srcMode == 1)
-            LCHK(launch_point_source(h->prec, h->W[0], h->W[1], h->W[2], h->num, h->nx_, h->ny_, h->SX, h->SY, h->SZ,
-                                     h->srcCoef, ricker_moment(h->fc, h->it, DT), h->stream));
+            LCHK(launch_point_source(h->prec, h->W[0], h->W[1], h->W[2], h->lay, h->SX, h->SY, h->SZ, h->srcCoef,
+                                     ricker_moment(h->fc, h->it, DT), h->stream));
         if (h->srcMode == 2 && h->multiActive) {
             // addMomenteRate (addMoment.cpp:238-264) at irk = 0
             const float tmpT = ((float)h->it + 0.0f) * DT;
@@ -531,7 +555,7 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
             } else {
                 const float t1 = float(srcIt) * h->sdt;
                 const float tw = (tmpT - t1) / h->sdt;
-                LCHK(launch_multi_source(h->prec, h->W[0], h->num, h->srcIdx, h->mrate, h->npts, srcIt, tw, DT,
+                LCHK(launch_multi_source(h->prec, h->W[0], h->lay, h->srcIdx, h->mrate, h->npts, srcIt, tw, DT,
                                          h->stream));
             }
         }
@@ -568,11 +592,11 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
             FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;
         }
         if (h->nst > 0 && h->it < h->d.NT)
-            LCHK(launch_station(h->prec, h->W[0], h->num, h->stIdx, h->nst, h->stWave, h->d.NT, h->it,
+            LCHK(launch_station(h->prec, h->W[0], h->lay, h->stIdx, h->nst, h->stWave, h->d.NT, h->it,
                                 (float)(1.0 / h->Cv), (float)(1.0 / h->Cs), h->stream));
         if (h->d.free_surface)
-            LCHK(launch_pgv(h->prec, h->W[0], h->num, h->nx, h->ny, h->nx_, h->ny_, h->nz_ - HALO - 1, h->pgv,
-                            (float)(1.0 / h->Cv), h->stream));
+            LCHK(launch_pgv(h->prec, h->W[0], h->lay, h->nx, h->ny, h->nz_ - HALO - 1, h->pgv, (float)(1.0 / h->Cv),
+                            h->stream));
         ++h->it;
     }
     return CGFDM_OK;
@@ -615,7 +639,7 @@ int cgfdm_get_slice(cgfdm_domain* h, int axis, int index, int var, float* out) {
     if (rc) return rc;
     h->bytes -= sizeof(float) * n;
     const float c = (float)(1.0 / (var < 3 ? h->Cv : h->Cs));
-    LCHK(launch_slice(h->prec, h->W[0], h->num, h->nx, h->ny, h->nz, h->nx_, h->ny_, axis, index, var, c, tmp, h->stream));
+    LCHK(launch_slice(h->prec, h->W[0], h->lay, h->nx, h->ny, h->nz, axis, index, var, c, tmp, h->stream));
     HIPCHK(hipMemcpyAsync(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->release(tmp);
@@ -630,7 +654,7 @@ int cgfdm_get_state(cgfdm_domain* h, float* out) {
     int rc = h->alloc((void**)&tmp, sizeof(float) * n);
     if (rc) return rc;
     h->bytes -= sizeof(float) * n;
-    LCHK(launch_gather(h->prec, h->W[0], h->num, h->nx, h->ny, h->nz, h->nx_, h->ny_, 9, tmp, h->stream));
+    LCHK(launch_gather(h->prec, h->W[0], h->lay, 9, h->nx, h->ny, h->nz, tmp, h->stream));
     HIPCHK(hipMemcpyAsync(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->release(tmp);
@@ -646,7 +670,7 @@ int cgfdm_set_state(cgfdm_domain* h, const float* in) {
     if (rc) return rc;
     h->bytes -= sizeof(float) * n;
     HIPCHK(hipMemcpyAsync(tmp, in, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
-    LCHK(launch_scatter(h->prec, h->W[0], h->num, h->nx, h->ny, h->nz, h->nx_, h->ny_, 9, tmp, h->stream));
+    LCHK(launch_scatter(h->prec, h->W[0], h->lay, 9, h->nx, h->ny, h->nz, tmp, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->release(tmp);
     return CGFDM_OK;
@@ -655,12 +679,12 @@ int cgfdm_set_state(cgfdm_domain* h, const float* in) {
 int cgfdm_get_cjm(cgfdm_domain* h, float* out) {
     if (!h || !out) { set_err("null argument"); return CGFDM_EINVAL; }
     HIPCHK(hipSetDevice(h->dev));
-    const long long n = h->num * 13;
+    const long long n = (long long)h->nx_ * h->ny_ * h->nz_ * 13;
     float* tmp = nullptr;
     int rc = h->alloc((void**)&tmp, sizeof(float) * n);
     if (rc) return rc;
     h->bytes -= sizeof(float) * n;
-    LCHK(launch_decode(h->prec, h->cjm, tmp, n, h->stream));
+    LCHK(launch_unpack(h->prec, h->cjm, h->lay, 13, tmp, h->stream));
     HIPCHK(hipMemcpyAsync(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->release(tmp);

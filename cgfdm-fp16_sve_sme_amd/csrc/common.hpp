@@ -30,6 +30,67 @@ template <> struct SV<__bf16> {
     __device__ __forceinline__ static __bf16 st(float v) { return (__bf16)v; }
 };
 
+// Internal memory layout of every 3-D array: k-major with the variables of one z-plane stored
+// consecutively, [k][var][j][i], rows padded to `px` elements and shifted right by `lead` so that
+// the first column of the bulk box starts on a 64-byte boundary. A plane of one variable is
+// `plane` = px * ny_ elements. For a given (k, var) the wave-uniform part of an address is
+// (k*V + var)*plane and the per-lane part is the 32-bit column offset j*px + i + lead.
+struct Layout {
+    int nx_, ny_, nz_;  // padded sizes (interior + 2*HALO)
+    int px, lead;       // row pitch and column shift, in elements
+    long long plane;    // px * ny_
+    __host__ __device__ __forceinline__ long long col(int i, int j) const { return (long long)j * px + i + lead; }
+    __host__ __device__ __forceinline__ long long at(int V, int v, int i, int j, int k) const {
+        return ((long long)k * V + v) * plane + col(i, j);
+    }
+    __host__ __device__ __forceinline__ long long size(int V) const { return (long long)nz_ * V * plane; }
+};
+
+// Global access with a wave-uniform base pointer and a 32-bit byte offset per lane, so the load
+// uses the scalar-base addressing mode instead of a 64-bit address per variable in VGPRs.
+// Every array here is below 4 GiB per variable plane (padded grid < 2^31 points, <= 4 bytes).
+template <class S>
+__device__ __forceinline__ float gld(const S* __restrict__ base, unsigned elem) {
+    return SV<S>::ld(*(const S*)((const char*)base + elem * (unsigned)sizeof(S)));
+}
+template <class S>
+__device__ __forceinline__ void gst(S* __restrict__ base, unsigned elem, float v) {
+    *(S*)((char*)base + elem * (unsigned)sizeof(S)) = SV<S>::st(v);
+}
+
+// Buffer-resource access: a 128-bit descriptor (base, byte size) in SGPRs, a per-lane 32-bit byte
+// offset and a wave-uniform byte offset, so a load or store needs no VALU address arithmetic.
+// Accesses past the descriptor's size return 0 (loads) or are dropped (stores).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
+
+template <class S>
+__device__ __forceinline__ Rsrc make_rsrc(const S* base, long long nelem) {
+    const long long bytes = nelem * (long long)sizeof(S);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL),
+                                             kRsrcWord3);
+}
+template <class S> __device__ __forceinline__ float bld(Rsrc r, unsigned voff, int soff);
+template <> __device__ __forceinline__ float bld<float>(Rsrc r, unsigned voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+}
+template <> __device__ __forceinline__ float bld<_Float16>(Rsrc r, unsigned voff, int soff) {
+    return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0));
+}
+template <> __device__ __forceinline__ float bld<__bf16>(Rsrc r, unsigned voff, int soff) {
+    return (float)__builtin_bit_cast(__bf16, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0));
+}
+template <class S> __device__ __forceinline__ void bst(Rsrc r, unsigned voff, int soff, float v);
+template <> __device__ __forceinline__ void bst<float>(Rsrc r, unsigned voff, int soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, soff, 0);
+}
+template <> __device__ __forceinline__ void bst<_Float16>(Rsrc r, unsigned voff, int soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, (int)voff, soff, 0);
+}
+template <> __device__ __forceinline__ void bst<__bf16>(Rsrc r, unsigned voff, int soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)v), r, (int)voff, soff, 0);
+}
+
 // One PML slab: auxiliary variables of the 9 fields, SoA [9][n], FP32, three RK registers
 // (current value, stage-0 copy, accumulator). Slab order x1,y1,z1,x2,y2,z2 as AUX6SURF
 // (reference struct.h:565-574).
@@ -54,22 +115,27 @@ struct StageParams {
     const float* pa[3];  // PML alpha per axis (padded length)
     const float* pb[3];  // 1/beta
     const float* pd[3];  // d/beta
-    const float* rdz;    // [2][nx_*ny_][9]: -rDZ*DX then -rDZ*DY
+    const float* rdz;    // [2][nx_*ny_][9]: -rDZ*DX then -rDZ*DY, indexed by i + j*nx_
     Slab slab[6];
-    long long num;       // padded points per variable
+    Layout lay;
     int nx, ny, nz;      // interior sizes
     int nx_, ny_;        // padded sizes
     int nPML, fs, pml, irk;
     int FB1, FB2, FB3;
     float rDH, DT, Crho;
-    // bulk box (handled by the fast kernel); the general kernel skips it when bulk_on
+    // bulk box [bi0,bi1)x[bj0,bj1)x[bk0,bk1): no PML slab, no free-surface layer; the fast
+    // kernel covers it and the general kernel covers the rest as up to six boxes
     int bulk_on;
+    int bulk_cfg, bulk_kc;  // tile shape index and z-chunk length (0: automatic)
     int bi0, bi1, bj0, bj1, bk0, bk1;
+    int nbox;
+    int box[6][6];          // i0,i1,j0,j1,k0,k1
+    long long boxStart[7];  // prefix sums of box sizes
 };
 
 struct SetupParams {
     int nx, ny, nz, nx_, ny_, nz_;
-    long long num;
+    Layout lay;
     double DH;
     int use_terrain;
     double depth;
@@ -91,22 +157,25 @@ int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 
 int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s);
 
-int launch_point_source(int prec, void* A, void* B, void* C, long long num, int nx_, int ny_, int SX, int SY,
-                        int SZ, const float* coef, float sval, hipStream_t s);
-int launch_multi_source(int prec, void* A, long long num, const long long* idx, const float* mrate,
-                        long long npts, int srcIt, float tw, float DT, hipStream_t s);
-int launch_scale_mrate(int prec, const void* cjm, long long num, const long long* idx, float* mrate,
-                       long long npts, int nt, float DH, float Cv, hipStream_t s);
-int launch_station(int prec, const void* A, long long num, const long long* idx, int nst, float* wave, int NT,
-                   int it, float c1, float c2, hipStream_t s);
-int launch_pgv(int prec, const void* A, long long num, int nx, int ny, int nx_, int ny_, int k, float* pgv,
-               float c, hipStream_t s);
-int launch_slice(int prec, const void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int axis,
-                 int index, int var, float c, float* out, hipStream_t s);
-int launch_gather(int prec, const void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                  float* out, hipStream_t s);
-int launch_scatter(int prec, void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                   const float* in, hipStream_t s);
+// Point lists (sources, stations) carry padded (i, j, k) triples.
+int launch_point_source(int prec, void* A, void* B, void* C, const Layout& L, int SX, int SY, int SZ,
+                        const float* coef, float sval, hipStream_t s);
+int launch_multi_source(int prec, void* A, const Layout& L, const int* ijk, const float* mrate, long long npts,
+                        int srcIt, float tw, float DT, hipStream_t s);
+int launch_scale_mrate(int prec, const void* cjm, const Layout& L, const int* ijk, float* mrate, long long npts,
+                       int nt, float DH, float Cv, hipStream_t s);
+int launch_station(int prec, const void* A, const Layout& L, const int* ijk, int nst, float* wave, int NT, int it,
+                   float c1, float c2, hipStream_t s);
+int launch_pgv(int prec, const void* A, const Layout& L, int nx, int ny, int k, float* pgv, float c, hipStream_t s);
+int launch_slice(int prec, const void* A, const Layout& L, int nx, int ny, int nz, int axis, int index, int var,
+                 float c, float* out, hipStream_t s);
+// interior <-> float array of shape (V, NZ, NY, NX)
+int launch_gather(int prec, const void* A, const Layout& L, int V, int nx, int ny, int nz, float* out,
+                  hipStream_t s);
+int launch_scatter(int prec, void* A, const Layout& L, int V, int nx, int ny, int nz, const float* in,
+                   hipStream_t s);
+// whole padded array -> float array in the reference order (V, nz_, ny_, nx_)
+int launch_unpack(int prec, const void* src, const Layout& L, int V, float* dst, hipStream_t s);
 int launch_decode(int prec, const void* src, float* dst, long long n, hipStream_t s);
 
 inline int storage_bytes(int prec) { return prec == P_F32 ? 4 : 2; }

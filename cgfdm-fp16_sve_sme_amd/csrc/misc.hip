@@ -1,4 +1,5 @@
 // misc.hip — per-step side kernels: sources, stations, PGV, slices and state transfer.
+// Point lists carry padded (i, j, k) triples; addresses come from the Layout.
 #include "common.hpp"
 
 namespace cg {
@@ -11,18 +12,16 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 // the host; sval is the Ricker moment-rate value of this step. Points in the halo are also
 // added to the two stage buffers, whose halos must equal W's (they are never recomputed).
 template <class S>
-__global__ void k_point_source(S* A, S* B, S* C, long long num, int nx_, int ny_, int SX, int SY, int SZ,
-                               const float* __restrict__ coef, float sval) {
+__global__ void k_point_source(S* A, S* B, S* C, Layout L, int SX, int SY, int SZ, const float* __restrict__ coef,
+                               float sval) {
     const int q = threadIdx.x;
     if (q >= 343) return;
     const int di = q % 7 - 3, dj = (q / 7) % 7 - 3, dk = q / 49 - 3;
     const int i = SX + di, j = SY + dj, k = SZ + dk;
-    const long long idx = i + (long long)j * nx_ + (long long)k * nx_ * ny_;
     const float value = sval * coef[q];
-    const int nz_ = (int)(num / ((long long)nx_ * ny_));
-    const bool halo = i < HALO || i >= nx_ - HALO || j < HALO || j >= ny_ - HALO || k < HALO || k >= nz_ - HALO;
+    const bool halo = i < HALO || i >= L.nx_ - HALO || j < HALO || j >= L.ny_ - HALO || k < HALO || k >= L.nz_ - HALO;
     for (int v = 3; v <= 5; ++v) {
-        const long long p = v * num + idx;
+        const long long p = L.at(9, v, i, j, k);
         A[p] = SV<S>::st(SV<S>::ld(A[p]) + value);
         if (halo) {
             B[p] = SV<S>::st(SV<S>::ld(B[p]) + value);
@@ -34,16 +33,16 @@ __global__ void k_point_source(S* A, S* B, S* C, long long num, int nx_, int ny_
 // Linear-in-time interpolation of the moment-rate table and point injection into the six
 // stresses (interp_momentRate + addSource1, addMoment.cpp:98-144, 194-219).
 template <class S>
-__global__ void k_multi_source(S* A, long long num, const long long* __restrict__ idx,
-                               const float* __restrict__ mrate, long long npts, int srcIt, float tw, float DT) {
+__global__ void k_multi_source(S* A, Layout L, const int* __restrict__ ijk, const float* __restrict__ mrate,
+                               long long npts, int srcIt, float tw, float DT) {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (p >= npts) return;
-    const long long id = idx[p];
+    const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
     const float* m0 = mrate + ((long long)srcIt * npts + p) * 6;
     const float* m1 = mrate + ((long long)(srcIt + 1) * npts + p) * 6;
     for (int c = 0; c < 6; ++c) {
         const float slice = (m1[c] - m0[c]) * tw + m0[c];
-        const long long q = (3 + c) * num + id;
+        const long long q = L.at(9, 3 + c, i, j, k);
         A[q] = SV<S>::st(SV<S>::ld(A[q]) - slice * DT);
     }
 }
@@ -51,14 +50,14 @@ __global__ void k_multi_source(S* A, long long num, const long long* __restrict_
 // One-time scaling of the table by mu*Cv/(J*DH^3) read from the stored CJM
 // (calculate_MomentRate, addMoment.cpp:11-65).
 template <class S>
-__global__ void k_scale_mrate(const S* __restrict__ cjm, long long num, const long long* __restrict__ idx,
-                              float* mrate, long long npts, int nt, float DH, float Cv) {
+__global__ void k_scale_mrate(const S* __restrict__ cjm, Layout L, const int* __restrict__ ijk, float* mrate,
+                              long long npts, int nt, float DH, float Cv) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= npts * nt) return;
     const long long p = t % npts;
-    const long long id = idx[p];
-    const float jacb = SV<S>::ld(cjm[9 * num + id]);
-    const float mu = SV<S>::ld(cjm[10 * num + id]);
+    const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
+    const float jacb = SV<S>::ld(cjm[L.at(13, 9, i, j, k)]);
+    const float mu = SV<S>::ld(cjm[L.at(13, 10, i, j, k)]);
     float V = jacb * DH * DH * DH;
     V = 1.0f / V;
     V = V * Cv;
@@ -67,24 +66,24 @@ __global__ void k_scale_mrate(const S* __restrict__ cjm, long long num, const lo
 
 // Station sampling after the step (storage_station, station.cpp:309-347).
 template <class S>
-__global__ void k_station(const S* __restrict__ A, long long num, const long long* __restrict__ idx, int nst,
-                          float* wave, int NT, int it, float c1, float c2) {
+__global__ void k_station(const S* __restrict__ A, Layout L, const int* __restrict__ ijk, int nst, float* wave,
+                          int NT, int it, float c1, float c2) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nst * 9) return;
     const int s = t / 9, v = t % 9;
     const long long pos = it + (long long)s * NT;
-    wave[pos * 9 + v] = SV<S>::ld(A[v * num + idx[s]]) * (v < 3 ? c1 : c2);
+    wave[pos * 9 + v] = SV<S>::ld(A[L.at(9, v, ijk[3 * s], ijk[3 * s + 1], ijk[3 * s + 2])]) * (v < 3 ? c1 : c2);
 }
 
 // Running maximum of horizontal and total speed on the surface layer (compare_pgv, PGV.cpp:48-101).
 template <class S>
-__global__ void k_pgv(const S* __restrict__ A, long long num, int nx, int ny, int nx_, int ny_, int k,
-                      float* __restrict__ pgv, float c) {
+__global__ void k_pgv(const S* __restrict__ A, Layout L, int nx, int ny, int k, float* __restrict__ pgv, float c) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx * ny) return;
-    const int i0 = t % nx, j0 = t / nx;
-    const long long idx = (i0 + HALO) + (long long)(j0 + HALO) * nx_ + (long long)k * nx_ * ny_;
-    const float Vx = SV<S>::ld(A[idx]) * c, Vy = SV<S>::ld(A[num + idx]) * c, Vz = SV<S>::ld(A[2 * num + idx]) * c;
+    const int i = t % nx + HALO, j = t / nx + HALO;
+    const float Vx = SV<S>::ld(A[L.at(9, 0, i, j, k)]) * c;
+    const float Vy = SV<S>::ld(A[L.at(9, 1, i, j, k)]) * c;
+    const float Vz = SV<S>::ld(A[L.at(9, 2, i, j, k)]) * c;
     const float Vh = sqrtf(Vx * Vx + Vy * Vy);
     const float V = sqrtf(Vx * Vx + Vy * Vy + Vz * Vz);
     if (pgv[t * 2 + 0] < Vh) pgv[t * 2 + 0] = Vh;
@@ -93,8 +92,8 @@ __global__ void k_pgv(const S* __restrict__ A, long long num, int nx, int ny, in
 
 // Plane gather with unscaling (pack_iodata_x/y/z, data_io.cpp:86-231).
 template <class S>
-__global__ void k_slice(const S* __restrict__ A, long long num, int nx, int ny, int nz, int nx_, int ny_, int axis,
-                        int index, int var, float c, float* __restrict__ out) {
+__global__ void k_slice(const S* __restrict__ A, Layout L, int nx, int ny, int nz, int axis, int index, int var,
+                        float c, float* __restrict__ out) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k;
     long long n;
@@ -102,32 +101,41 @@ __global__ void k_slice(const S* __restrict__ A, long long num, int nx, int ny, 
     else if (axis == 1) { n = (long long)nx * nz; j = index; i = (int)(t % nx); k = (int)(t / nx); }
     else { n = (long long)nx * ny; k = index; i = (int)(t % nx); j = (int)(t / nx); }
     if (t >= n) return;
-    const long long idx = (i + HALO) + (long long)(j + HALO) * nx_ + (long long)(k + HALO) * nx_ * ny_;
-    out[t] = SV<S>::ld(A[var * num + idx]) * c;
+    out[t] = SV<S>::ld(A[L.at(9, var, i + HALO, j + HALO, k + HALO)]) * c;
 }
 
 template <class S>
-__global__ void k_gather(const S* __restrict__ A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                         float* __restrict__ out) {
+__global__ void k_gather(const S* __restrict__ A, Layout L, int V, int nx, int ny, int nz, float* __restrict__ out) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long n = (long long)nx * ny * nz;
-    if (t >= n * nvar) return;
+    if (t >= n * V) return;
     const int v = (int)(t / n);
     const long long r = t % n;
     const int i = (int)(r % nx), j = (int)((r / nx) % ny), k = (int)(r / ((long long)nx * ny));
-    out[t] = SV<S>::ld(A[v * num + (i + HALO) + (long long)(j + HALO) * nx_ + (long long)(k + HALO) * nx_ * ny_]);
+    out[t] = SV<S>::ld(A[L.at(V, v, i + HALO, j + HALO, k + HALO)]);
 }
 
 template <class S>
-__global__ void k_scatter(S* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                          const float* __restrict__ in) {
+__global__ void k_scatter(S* A, Layout L, int V, int nx, int ny, int nz, const float* __restrict__ in) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long n = (long long)nx * ny * nz;
-    if (t >= n * nvar) return;
+    if (t >= n * V) return;
     const int v = (int)(t / n);
     const long long r = t % n;
     const int i = (int)(r % nx), j = (int)((r / nx) % ny), k = (int)(r / ((long long)nx * ny));
-    A[v * num + (i + HALO) + (long long)(j + HALO) * nx_ + (long long)(k + HALO) * nx_ * ny_] = SV<S>::st(in[t]);
+    A[L.at(V, v, i + HALO, j + HALO, k + HALO)] = SV<S>::st(in[t]);
+}
+
+// Whole padded array to the reference order (V, nz_, ny_, nx_).
+template <class S>
+__global__ void k_unpack(const S* __restrict__ A, Layout L, int V, float* __restrict__ out) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n = (long long)L.nx_ * L.ny_ * L.nz_;
+    if (t >= n * V) return;
+    const int v = (int)(t / n);
+    const long long r = t % n;
+    const int i = (int)(r % L.nx_), j = (int)((r / L.nx_) % L.ny_), k = (int)(r / ((long long)L.nx_ * L.ny_));
+    out[t] = SV<S>::ld(A[L.at(V, v, i, j, k)]);
 }
 
 template <class S>
@@ -144,66 +152,71 @@ __global__ void k_decode(const S* __restrict__ src, float* __restrict__ dst, lon
         default: return -1;                                                                           \
     }
 
-int launch_point_source(int prec, void* A, void* B, void* C, long long num, int nx_, int ny_, int SX, int SY, int SZ,
+int launch_point_source(int prec, void* A, void* B, void* C, const Layout& L, int SX, int SY, int SZ,
                         const float* coef, float sval, hipStream_t s) {
     switch (prec) {
-        case P_F32: k_point_source<float><<<1, 384, 0, s>>>((float*)A, (float*)B, (float*)C, num, nx_, ny_, SX, SY, SZ, coef, sval); break;
-        case P_F16: k_point_source<_Float16><<<1, 384, 0, s>>>((_Float16*)A, (_Float16*)B, (_Float16*)C, num, nx_, ny_, SX, SY, SZ, coef, sval); break;
-        case P_BF16: k_point_source<__bf16><<<1, 384, 0, s>>>((__bf16*)A, (__bf16*)B, (__bf16*)C, num, nx_, ny_, SX, SY, SZ, coef, sval); break;
+        case P_F32: k_point_source<float><<<1, 384, 0, s>>>((float*)A, (float*)B, (float*)C, L, SX, SY, SZ, coef, sval); break;
+        case P_F16: k_point_source<_Float16><<<1, 384, 0, s>>>((_Float16*)A, (_Float16*)B, (_Float16*)C, L, SX, SY, SZ, coef, sval); break;
+        case P_BF16: k_point_source<__bf16><<<1, 384, 0, s>>>((__bf16*)A, (__bf16*)B, (__bf16*)C, L, SX, SY, SZ, coef, sval); break;
         default: return -1;
     }
     return CG_LAUNCH_OK;
 }
 
-int launch_multi_source(int prec, void* A, long long num, const long long* idx, const float* mrate, long long npts,
+int launch_multi_source(int prec, void* A, const Layout& L, const int* ijk, const float* mrate, long long npts,
                         int srcIt, float tw, float DT, hipStream_t s) {
-    CG_DISPATCH(prec, k_multi_source, nblk(npts, 256), 256, s, (float*)A, (_Float16*)A, (__bf16*)A, num, idx, mrate,
+    CG_DISPATCH(prec, k_multi_source, nblk(npts, 256), 256, s, (float*)A, (_Float16*)A, (__bf16*)A, L, ijk, mrate,
                 npts, srcIt, tw, DT);
     return CG_LAUNCH_OK;
 }
 
-int launch_scale_mrate(int prec, const void* cjm, long long num, const long long* idx, float* mrate, long long npts,
+int launch_scale_mrate(int prec, const void* cjm, const Layout& L, const int* ijk, float* mrate, long long npts,
                        int nt, float DH, float Cv, hipStream_t s) {
     CG_DISPATCH(prec, k_scale_mrate, nblk(npts * nt, 256), 256, s, (const float*)cjm, (const _Float16*)cjm,
-                (const __bf16*)cjm, num, idx, mrate, npts, nt, DH, Cv);
+                (const __bf16*)cjm, L, ijk, mrate, npts, nt, DH, Cv);
     return CG_LAUNCH_OK;
 }
 
-int launch_station(int prec, const void* A, long long num, const long long* idx, int nst, float* wave, int NT, int it,
+int launch_station(int prec, const void* A, const Layout& L, const int* ijk, int nst, float* wave, int NT, int it,
                    float c1, float c2, hipStream_t s) {
     CG_DISPATCH(prec, k_station, nblk((long long)nst * 9, 256), 256, s, (const float*)A, (const _Float16*)A,
-                (const __bf16*)A, num, idx, nst, wave, NT, it, c1, c2);
+                (const __bf16*)A, L, ijk, nst, wave, NT, it, c1, c2);
     return CG_LAUNCH_OK;
 }
 
-int launch_pgv(int prec, const void* A, long long num, int nx, int ny, int nx_, int ny_, int k, float* pgv, float c,
-               hipStream_t s) {
+int launch_pgv(int prec, const void* A, const Layout& L, int nx, int ny, int k, float* pgv, float c, hipStream_t s) {
     CG_DISPATCH(prec, k_pgv, nblk((long long)nx * ny, 256), 256, s, (const float*)A, (const _Float16*)A,
-                (const __bf16*)A, num, nx, ny, nx_, ny_, k, pgv, c);
+                (const __bf16*)A, L, nx, ny, k, pgv, c);
     return CG_LAUNCH_OK;
 }
 
-int launch_slice(int prec, const void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int axis, int index,
-                 int var, float c, float* out, hipStream_t s) {
+int launch_slice(int prec, const void* A, const Layout& L, int nx, int ny, int nz, int axis, int index, int var,
+                 float c, float* out, hipStream_t s) {
     long long n = axis == 0 ? (long long)ny * nz : (axis == 1 ? (long long)nx * nz : (long long)nx * ny);
-    CG_DISPATCH(prec, k_slice, nblk(n, 256), 256, s, (const float*)A, (const _Float16*)A, (const __bf16*)A, num, nx,
-                ny, nz, nx_, ny_, axis, index, var, c, out);
+    CG_DISPATCH(prec, k_slice, nblk(n, 256), 256, s, (const float*)A, (const _Float16*)A, (const __bf16*)A, L, nx,
+                ny, nz, axis, index, var, c, out);
     return CG_LAUNCH_OK;
 }
 
-int launch_gather(int prec, const void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                  float* out, hipStream_t s) {
-    long long n = (long long)nx * ny * nz * nvar;
-    CG_DISPATCH(prec, k_gather, nblk(n, 256), 256, s, (const float*)A, (const _Float16*)A, (const __bf16*)A, num, nx,
-                ny, nz, nx_, ny_, nvar, out);
+int launch_gather(int prec, const void* A, const Layout& L, int V, int nx, int ny, int nz, float* out,
+                  hipStream_t s) {
+    long long n = (long long)nx * ny * nz * V;
+    CG_DISPATCH(prec, k_gather, nblk(n, 256), 256, s, (const float*)A, (const _Float16*)A, (const __bf16*)A, L, V,
+                nx, ny, nz, out);
     return CG_LAUNCH_OK;
 }
 
-int launch_scatter(int prec, void* A, long long num, int nx, int ny, int nz, int nx_, int ny_, int nvar,
-                   const float* in, hipStream_t s) {
-    long long n = (long long)nx * ny * nz * nvar;
-    CG_DISPATCH(prec, k_scatter, nblk(n, 256), 256, s, (float*)A, (_Float16*)A, (__bf16*)A, num, nx, ny, nz, nx_,
-                ny_, nvar, in);
+int launch_scatter(int prec, void* A, const Layout& L, int V, int nx, int ny, int nz, const float* in,
+                   hipStream_t s) {
+    long long n = (long long)nx * ny * nz * V;
+    CG_DISPATCH(prec, k_scatter, nblk(n, 256), 256, s, (float*)A, (_Float16*)A, (__bf16*)A, L, V, nx, ny, nz, in);
+    return CG_LAUNCH_OK;
+}
+
+int launch_unpack(int prec, const void* src, const Layout& L, int V, float* dst, hipStream_t s) {
+    long long n = (long long)L.nx_ * L.ny_ * L.nz_ * V;
+    CG_DISPATCH(prec, k_unpack, nblk(n, 256), 256, s, (const float*)src, (const _Float16*)src, (const __bf16*)src,
+                L, V, dst);
     return CG_LAUNCH_OK;
 }
 

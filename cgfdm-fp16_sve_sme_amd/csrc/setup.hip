@@ -79,12 +79,11 @@ __global__ void k_medium(SetupParams q, const float* __restrict__ vs, const floa
     long long n = (long long)q.nx * q.ny * q.nz;
     if (t >= n) return;
     int i = (int)(t % q.nx), j = (int)((t / q.nx) % q.ny), k = (int)(t / ((long long)q.nx * q.ny));
-    long long idx = (i + HALO) + (long long)(j + HALO) * q.nx_ + (long long)(k + HALO) * q.nx_ * q.ny_;
     double mu, lambda, buoy;
     medium_at(q, vs[t], vp[t], rho[t], mu, lambda, buoy);
-    cjm[10 * q.num + idx] = SV<S>::st((float)mu);
-    cjm[11 * q.num + idx] = SV<S>::st((float)lambda);
-    cjm[12 * q.num + idx] = SV<S>::st((float)buoy);
+    cjm[q.lay.at(13, 10, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)mu);
+    cjm[q.lay.at(13, 11, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)lambda);
+    cjm[q.lay.at(13, 12, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)buoy);
 }
 
 // Metric and J on the interior and, with a free surface, the mirrored values above it
@@ -101,9 +100,8 @@ __global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
     int kk = k < _nz ? k : 2 * (_nz - 1) - k;
     double m[9], J;
     metric_at(q, i, j, kk, m, J);
-    long long idx = i + (long long)j * q.nx_ + (long long)k * q.nx_ * q.ny_;
-    for (int c = 0; c < 9; ++c) cjm[c * q.num + idx] = SV<S>::st((float)m[c]);
-    cjm[9 * q.num + idx] = SV<S>::st((float)J);
+    for (int c = 0; c < 9; ++c) cjm[q.lay.at(13, c, i, j, k)] = SV<S>::st((float)m[c]);
+    cjm[q.lay.at(13, 9, i, j, k)] = SV<S>::st((float)J);
 }
 
 // Free-surface matrices -rDZ^-1 DX and -rDZ^-1 DY at the surface layer

@@ -29,16 +29,16 @@ __device__ __forceinline__ float LJT(const float* J_T, int FB, float rDH) {
            (af_1 * J_T[3 - FB] + af0 * J_T[3] + af1 * J_T[3 + FB] + af2 * J_T[3 + 2 * FB] + af3 * J_T[3 + 3 * FB]) * rDH;
 }
 
-// RK register update of the 9 wavefield variables at idx (wave_rk0..3).
+// RK register update of the 9 wavefield variables at element idx of variable 0 (wave_rk0..3).
 template <class S>
 __device__ __forceinline__ void rk_update(const StageParams& p, long long idx, const float* h) {
     S* __restrict__ Wo = (S*)p.Wout;
     const S* __restrict__ A = (const S*)p.A;
     S* __restrict__ T = (S*)p.T;
-    const long long num = p.num;
+    const long long plane = p.lay.plane;
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-        const long long q = v * num + idx;
+        const long long q = v * plane + idx;
         if (p.irk == 0) {
             float m = SV<S>::ld(A[q]);
             Wo[q] = SV<S>::st(m + alpha2 * h[v]);
@@ -62,27 +62,33 @@ template <class S>
 __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
     const S* __restrict__ W = (const S*)p.Win;
     const S* __restrict__ Cj = (const S*)p.cjm;
-    const long long num = p.num;
-    const long long sx = 1, sy = p.nx_, sz = (long long)p.nx_ * p.ny_;
-    const long long idx = i + j * sy + k * sz;
+    // W element (v,i,j,k) = v*plane + idx (z stride 9*plane); CJM (c,i,j,k) = c*plane + idc
+    const long long plane = p.lay.plane;
+    const long long sy = p.lay.px;
+    const long long szW = 9 * plane, szC = 13 * plane;
+    const long long colij = p.lay.col(i, j);
+    const long long idx = colij + k * szW;
+    const long long idc = colij + k * szC;
+    const long long rdzi = (i + (long long)j * p.nx_) * 9;
     const float rDH = p.rDH, DT = p.DT;
     const int _nz = p.nz + HALO;
     const int FBs[3] = {p.FB1, p.FB2, p.FB3};
-    const long long strd[3] = {sx, sy, sz};
+    const long long sW[3] = {1, sy, szW};
+    const long long sC[3] = {1, sy, szC};
 
     float M[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) M[c] = SV<S>::ld(Cj[c * num + idx]);
-    const float mu = SV<S>::ld(Cj[10 * num + idx]);
-    const float lambda = SV<S>::ld(Cj[11 * num + idx]);
-    const float buoyancy = SV<S>::ld(Cj[12 * num + idx]) * p.Crho;
+    for (int c = 0; c < 9; ++c) M[c] = SV<S>::ld(Cj[c * plane + idc]);
+    const float mu = SV<S>::ld(Cj[10 * plane + idc]);
+    const float lambda = SV<S>::ld(Cj[11 * plane + idc]);
+    const float buoyancy = SV<S>::ld(Cj[12 * plane + idc]) * p.Crho;
 
     // raw one-sided derivatives D[dir][var] (no PML factor)
     float D[3][9];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int v = 0; v < 9; ++v) D[d][v] = Lraw<S>(W + v * num, idx, strd[d], FBs[d], rDH);
+        for (int v = 0; v < 9; ++v) D[d][v] = Lraw<S>(W + v * plane, idx, sW[d], FBs[d], rDH);
 
     const float bx = p.pb[0][i], by = p.pb[1][j], bz = p.pb[2][k];
     float h[9];
@@ -96,9 +102,6 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             B[1][v] = D[1][v] * by;
             B[2][v] = D[2][v] * bz;
         }
-        float hv[9];
-#pragma unroll
-        for (int v = 0; v < 9; ++v) hv[v] = 0.f;
         float Vx[3], Vy[3], Vz[3], Txx[3], Tyy[3], Tzz[3], Txy[3], Txz[3], Tyz[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -124,7 +127,6 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
         h[6] = (Txy[0] + Txy[1] + Txy[2]) * DT;
         h[7] = (Txz[0] + Txz[1] + Txz[2]) * DT;
         h[8] = (Tyz[0] + Tyz[1] + Tyz[2]) * DT;
-        (void)hv;
     } else {
         // freeSurface.cpp:104-256: conservative J*(metric . T) form for the velocity with
         // antisymmetric imaging of the stress about the surface
@@ -134,28 +136,30 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
         for (int l = 0; l <= 2 * HALO; ++l) {
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
-                const long long pos = idx + (l - HALO) * strd[d];
-                const float m0 = SV<S>::ld(Cj[(3 * d) * num + pos]);
-                const float m1 = SV<S>::ld(Cj[(3 * d + 1) * num + pos]);
-                const float m2 = SV<S>::ld(Cj[(3 * d + 2) * num + pos]);
-                const float jac = SV<S>::ld(Cj[9 * num + pos]);
-                const float w3 = SV<S>::ld(W[3 * num + pos]), w4 = SV<S>::ld(W[4 * num + pos]);
-                const float w5 = SV<S>::ld(W[5 * num + pos]), w6 = SV<S>::ld(W[6 * num + pos]);
-                const float w7 = SV<S>::ld(W[7 * num + pos]), w8 = SV<S>::ld(W[8 * num + pos]);
+                const long long pos = idx + (l - HALO) * sW[d];
+                const long long pc = idc + (l - HALO) * sC[d];
+                const float m0 = SV<S>::ld(Cj[(3 * d) * plane + pc]);
+                const float m1 = SV<S>::ld(Cj[(3 * d + 1) * plane + pc]);
+                const float m2 = SV<S>::ld(Cj[(3 * d + 2) * plane + pc]);
+                const float jac = SV<S>::ld(Cj[9 * plane + pc]);
+                const float w3 = SV<S>::ld(W[3 * plane + pos]), w4 = SV<S>::ld(W[4 * plane + pos]);
+                const float w5 = SV<S>::ld(W[5 * plane + pos]), w6 = SV<S>::ld(W[6 * plane + pos]);
+                const float w7 = SV<S>::ld(W[7 * plane + pos]), w8 = SV<S>::ld(W[8 * plane + pos]);
                 JT[d][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
                 JT[d][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
                 JT[d][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
             }
         }
         for (int l = 0; l < k_s; ++l) {
-            const long long pos = idx + (l - HALO) * sz;
-            const float m0 = SV<S>::ld(Cj[6 * num + pos]);
-            const float m1 = SV<S>::ld(Cj[7 * num + pos]);
-            const float m2 = SV<S>::ld(Cj[8 * num + pos]);
-            const float jac = SV<S>::ld(Cj[9 * num + pos]);
-            const float w3 = SV<S>::ld(W[3 * num + pos]), w4 = SV<S>::ld(W[4 * num + pos]);
-            const float w5 = SV<S>::ld(W[5 * num + pos]), w6 = SV<S>::ld(W[6 * num + pos]);
-            const float w7 = SV<S>::ld(W[7 * num + pos]), w8 = SV<S>::ld(W[8 * num + pos]);
+            const long long pos = idx + (l - HALO) * szW;
+            const long long pc = idc + (l - HALO) * szC;
+            const float m0 = SV<S>::ld(Cj[6 * plane + pc]);
+            const float m1 = SV<S>::ld(Cj[7 * plane + pc]);
+            const float m2 = SV<S>::ld(Cj[8 * plane + pc]);
+            const float jac = SV<S>::ld(Cj[9 * plane + pc]);
+            const float w3 = SV<S>::ld(W[3 * plane + pos]), w4 = SV<S>::ld(W[4 * plane + pos]);
+            const float w5 = SV<S>::ld(W[5 * plane + pos]), w6 = SV<S>::ld(W[6 * plane + pos]);
+            const float w7 = SV<S>::ld(W[7 * plane + pos]), w8 = SV<S>::ld(W[8 * plane + pos]);
             JT[2][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
             JT[2][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
             JT[2][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
@@ -164,7 +168,7 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             JT[2][c][k_s] = 0.f;
             for (int l = k_s + 1; l <= 2 * HALO; ++l) JT[2][c][l] = -JT[2][c][2 * k_s - l];
         }
-        const float jacb = SV<S>::ld(Cj[9 * num + idx]);
+        const float jacb = SV<S>::ld(Cj[9 * plane + idc]);
         const float Jinv = 1.0f / jacb;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -176,24 +180,24 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
         const float Vz_xi = D[0][2] * bx, Vz_et = D[1][2] * by;
         float Vx_zt, Vy_zt, Vz_zt;
         if (k == _nz - 1) {
-            const float* X = p.rdz + (i + j * sy) * 9;
-            const float* Y = p.rdz + (long long)p.nx_ * p.ny_ * 9 + (i + j * sy) * 9;
+            const float* X = p.rdz + rdzi;
+            const float* Y = p.rdz + (long long)p.nx_ * p.ny_ * 9 + rdzi;
             Vx_zt = (X[0] * Vx_xi + X[1] * Vy_xi + X[2] * Vz_xi) + (Y[0] * Vx_et + Y[1] * Vy_et + Y[2] * Vz_et);
             Vy_zt = (X[3] * Vx_xi + X[4] * Vy_xi + X[5] * Vz_xi) + (Y[3] * Vx_et + Y[4] * Vy_et + Y[5] * Vz_et);
             Vz_zt = (X[6] * Vx_xi + X[7] * Vy_xi + X[8] * Vz_xi) + (Y[6] * Vx_et + Y[7] * Vy_et + Y[8] * Vz_et);
         } else if (k == _nz - 2) {  // L2 (axis.h:92)
-            const long long s = p.FB3 * sz;
-            Vx_zt = (float)p.FB3 * (SV<S>::ld(W[0 * num + idx + s]) - SV<S>::ld(W[0 * num + idx])) * rDH;
-            Vy_zt = (float)p.FB3 * (SV<S>::ld(W[1 * num + idx + s]) - SV<S>::ld(W[1 * num + idx])) * rDH;
-            Vz_zt = (float)p.FB3 * (SV<S>::ld(W[2 * num + idx + s]) - SV<S>::ld(W[2 * num + idx])) * rDH;
+            const long long s = p.FB3 * szW;
+            Vx_zt = (float)p.FB3 * (SV<S>::ld(W[0 * plane + idx + s]) - SV<S>::ld(W[0 * plane + idx])) * rDH;
+            Vy_zt = (float)p.FB3 * (SV<S>::ld(W[1 * plane + idx + s]) - SV<S>::ld(W[1 * plane + idx])) * rDH;
+            Vz_zt = (float)p.FB3 * (SV<S>::ld(W[2 * plane + idx + s]) - SV<S>::ld(W[2 * plane + idx])) * rDH;
         } else {  // L3 (axis.h:93)
-            const long long s = p.FB3 * sz;
+            const long long s = p.FB3 * szW;
             float r3[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v)
                 r3[v] = (float)p.FB3 *
-                        (Cf1 * SV<S>::ld(W[v * num + idx]) + Cf2 * SV<S>::ld(W[v * num + idx + s]) +
-                         Cf3 * SV<S>::ld(W[v * num + idx + 2 * s])) *
+                        (Cf1 * SV<S>::ld(W[v * plane + idx]) + Cf2 * SV<S>::ld(W[v * plane + idx + s]) +
+                         Cf3 * SV<S>::ld(W[v * plane + idx + 2 * s])) *
                         rDH;
             Vx_zt = r3[0]; Vy_zt = r3[1]; Vz_zt = r3[2];
         }
@@ -255,7 +259,7 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             for (int v = 0; v < 9; ++v) ha[v] = (r[v] - alpha_d * aold[v]) * DT;
             if (p.fs && a < 2 && k == _nz - 1) {
                 // pml_free_surface_x/y (pml_freeSurface.cpp:81-116)
-                const float* Mz = p.rdz + (a == 0 ? 0 : (long long)p.nx_ * p.ny_ * 9) + (i + j * sy) * 9;
+                const float* Mz = p.rdz + (a == 0 ? 0 : (long long)p.nx_ * p.ny_ * 9) + rdzi;
                 const float zx = M[6], zy = M[7], zz = M[8];
                 const float Vz0 = Mz[0] * b[0] + Mz[1] * b[1] + Mz[2] * b[2];
                 const float Vz1 = Mz[3] * b[0] + Mz[4] * b[1] + Mz[5] * b[2];
