@@ -54,12 +54,13 @@ def source_and_stations(NX, NY, NZ, DH, DT, nt_src, rho_src, vs_src):
 def run_gpu(prec, grid, DH, DT, steps, warmup, model):
     NX, NY, NZ = grid
     terrain, depth, vs, vp, rho = model
-    d = cgfdm.make_desc(NX, NY, NZ, DH=DH, DT=DT, NT=steps + warmup, nPML=12, precision=prec,
+    NT = warmup + steps + min(steps, 4)
+    d = cgfdm.make_desc(NX, NY, NZ, DH=DH, DT=DT, NT=NT, nPML=12, precision=prec,
                         use_terrain=1, depth=depth)
     t0 = time.time()
     dom = cgfdm.Domain(d, device=0)
     dom.set_model(terrain, vs, vp, rho)
-    src, table, st = source_and_stations(NX, NY, NZ, DH, DT, steps + warmup + 2,
+    src, table, st = source_and_stations(NX, NY, NZ, DH, DT, NT + 2,
                                          rho[src_k(NZ, DH), NY // 2, NX // 2],
                                          vs[src_k(NZ, DH), NY // 2, NX // 2])
     dom.set_source_table(src, table, DT)
@@ -68,22 +69,29 @@ def run_gpu(prec, grid, DH, DT, steps, warmup, model):
     log(f"[{prec}] setup {time.time() - t0:.1f}s, device bytes {dom.bytes / 2**30:.1f} GiB")
     dom.step(warmup)
     dom.sync()
-    dom.enable_timing(True)
-    dom.stage_timing(reset=True)
+    # timed region: `steps` whole steps, no per-kernel events (they would add host work)
     t0 = time.perf_counter()
     dom.step(steps)
     dom.sync()
     wall = time.perf_counter() - t0
-    avg_ms, cnt = dom.stage_timing()
+    # kernel split over the same number of further steps, with HIP events on the domain's stream
+    dom.enable_timing(True)
+    dom.kernel_timing(reset=True)
+    dom.step(min(steps, 4))
+    (stage_ms, bulk_ms, gen_ms), cnt = dom.kernel_timing()
+    dom.enable_timing(False)
+    bulk_pts, gen_pts, tile, kc = dom.partition()
     sts = dom.stations()
     finite = bool(np.isfinite(sts).all())
     dom.close()
     n = NX * NY * NZ
     glups = n * steps / wall / 1e9
-    log(f"[{prec}] {steps} steps in {wall:.3f}s: {glups:.3f} GLUPS, stage kernel avg {avg_ms:.3f} ms "
-        f"over {cnt} launches, stations finite={finite}")
-    return {"glups": glups, "ms_per_step": wall / steps * 1e3, "stage_avg_ms": avg_ms,
-            "stage_launches": cnt, "finite": finite}
+    log(f"[{prec}] {steps} steps in {wall:.3f}s: {glups:.3f} GLUPS; per stage {stage_ms:.3f} ms = bulk "
+        f"{bulk_ms:.3f} ms ({bulk_pts / 1e6:.1f} M pts, tile {tile}, kc {kc}) + boundary {gen_ms:.3f} ms "
+        f"({gen_pts / 1e6:.1f} M pts), stations finite={finite}")
+    return {"glups": glups, "ms_per_step": wall / steps * 1e3, "stage_avg_ms": stage_ms,
+            "bulk_ms": bulk_ms, "boundary_ms": gen_ms, "bulk_points": bulk_pts, "boundary_points": gen_pts,
+            "tile_cfg": tile, "kc": kc, "stage_launches": cnt, "finite": finite}
 
 
 def src_k(NZ, DH):
@@ -122,6 +130,106 @@ def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=5):
                       f"{steps} timed steps, OpenMP over k on {threads} threads"}
 
 
+def serving_run(reqs, max_domains, steps_per_launch, paced=True):
+    """Serves `reqs` [(arrival_s, request)] with one cgfdm.Server; requests are submitted at
+    their arrival times by a second thread (paced) or all at once. Returns latencies (s, from
+    submission to results on the host), the makespan and the server counters."""
+    import threading
+    descs = []
+    for _, r in reqs:
+        NX, NY, NZ = r["grid"]
+        descs.append(cgfdm.make_desc(NX, NY, NZ, DH=r["DH"], DT=r["DT"], NT=r["NT"], nPML=r["nPML"],
+                                     precision=r["precision"]))
+    srv = cgfdm.Server(precision=reqs[0][1]["precision"], max_domains=max_domains,
+                       steps_per_launch=steps_per_launch, use_graph=True)
+    ids = {}
+    err = []
+
+    def feeder():
+        t0 = time.perf_counter()
+        for (t, r), d in zip(reqs, descs):
+            if paced:
+                dt = t - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            try:
+                ids[srv.submit({"desc": d, "layers": r["layers"], "table": r["table"],
+                                "stations": r["stations"]})] = d
+            except Exception as e:  # reported by the main thread
+                err.append(e)
+                return
+
+    th = threading.Thread(target=feeder)
+    t0 = time.perf_counter()
+    th.start()
+    lat, first, last = [], None, 0.0
+    done = 0
+    while done < len(reqs):
+        if err:
+            raise err[0]
+        got = srv.wait(timeout_ms=120000)
+        if not got:
+            raise RuntimeError(f"serving stalled: {srv.stats()}")
+        for rid in got:
+            rc, st, pgv, info = srv.result(rid)
+            if rc != 0:
+                raise RuntimeError(f"request {rid} failed: {info['error']}")
+            if not np.isfinite(st).all():
+                raise RuntimeError(f"request {rid}: non-finite seismograms")
+            lat.append(info["t_done"] - info["t_submit"])
+            first = info["t_submit"] if first is None else min(first, info["t_submit"])
+            last = max(last, info["t_done"])
+            done += 1
+    th.join()
+    stats = srv.stats()
+    srv.close()
+    wall = time.perf_counter() - t0
+    return np.array(lat), last - first, wall, stats
+
+
+def serving(n, seed, max_domains, steps_per_launch):
+    reqs = wl.serving_stream(n, seed)
+    lups = sum(r["grid"][0] * r["grid"][1] * r["grid"][2] * r["NT"] for _, r in reqs)
+    out = {}
+    for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
+        lat, span, wall, st = serving_run(reqs, md, steps_per_launch)
+        out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
+                     "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
+                     "glups": round(lups / span / 1e9, 3), "peak_resident": st["peak_resident"],
+                     "launches": st["launches"]}
+        log(f"[serving {name}] {n} requests in {span:.1f}s: {n / span:.3f} solves/s, p50 "
+            f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s, {lups / span / 1e9:.2f} GLUPS, "
+            f"peak resident {st['peak_resident']}")
+    out["requests"] = n
+    out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at 20/s, "
+                     f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
+                     f"couples 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface stations")
+    out["speedup"] = round(out["batched"]["solves_per_s"] / out["one_at_a_time"]["solves_per_s"], 3)
+    return out
+
+
+def copy_probe(gib):
+    """A plain device copy of known size, for calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = int(gib * 2**30)
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 copies of {n} bytes")
+
+
+def measured_traffic(prec, grid):
+    """HBM bytes per bulk-kernel launch from the committed counter summary of this workload, if
+    one exists (profiles/traffic_*.json, written from two rocprofv3 --pmc runs)."""
+    fn = os.path.join(ROOT, "profiles", f"traffic_bulk_{prec}_{grid[0]}x{grid[1]}x{grid[2]}.json")
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,62 +239,78 @@ def main():
     ap.add_argument("--precisions", default="fp16,fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving stream to run (0: skip; 512: the whole stream)")
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-steps-per-launch", type=int, default=8)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
               f"decomposition is not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     grid = tuple(int(x) for x in args.grid.split(","))
     DH, DT = 100.0, 0.005
     NX, NY, NZ = grid
-    t0 = time.time()
-    model = build_model(NX, NY, NZ, DH)
-    log(f"model {NX}x{NY}x{NZ} built in {time.time() - t0:.1f}s")
     res = {}
-    for prec in args.precisions.split(","):
-        res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
-    del model
-    head = "fp16" if "fp16" in res else list(res)[0]
-    r = res[head]
-    sb = 4 if head == "fp32" else 2
-    n = NX * NY * NZ
-    bytes_per_launch = VALUES_PER_LUP / 4 * sb * n
-    achieved = bytes_per_launch / (r["stage_avg_ms"] * 1e-3) / 1e9 if r["stage_avg_ms"] > 0 else None
+    precs = [p for p in args.precisions.split(",") if p]
+    if precs:
+        t0 = time.time()
+        model = build_model(NX, NY, NZ, DH)
+        log(f"model {NX}x{NY}x{NZ} built in {time.time() - t0:.1f}s")
+        for prec in precs:
+            res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
+        del model
+    serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch) \
+        if args.serve_requests > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(DH, DT, steps=args.cpu_steps)
-    out = {
-        "metric": METRIC,
-        "value": round(r["glups"], 4),
-        "unit": "GLUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r["ms_per_step"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"CGFDM time loop {NX}x{NY}x{NZ}, dh 100 m, DT 5 ms, random terrain "
-                               f"1 km RMS, layered medium + 5% random, double-couple table source, "
-                               f"21x21 stations, PGV, nPML 12, free surface",
-                   "grid": [NX, NY, NZ], "storage": head, "arith": "f32"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
-                     "traffic": None,
-                     "kernel": "fused RK stage (deriv + free surface + PML + RK update)",
-                     "bytes_per_launch": bytes_per_launch},
-        "cpu_baseline": cpu,
-    }
+    head = "fp16" if "fp16" in res else (list(res)[0] if res else None)
+    out = {"metric": METRIC, "unit": "GLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic"}
+    if head:
+        r = res[head]
+        sb = 4 if head == "fp32" else 2
+        # dominant kernel: the bulk stage kernel; 48 values per point per launch on average
+        bulk_bytes = VALUES_PER_LUP / 4 * sb * r["bulk_points"]
+        achieved = bulk_bytes / (r["bulk_ms"] * 1e-3) / 1e9 if r["bulk_ms"] > 0 else None
+        tr = measured_traffic(head, grid)
+        out.update({
+            "value": round(r["glups"], 4),
+            "ms_per_step": round(r["ms_per_step"], 3),
+            "config": {"workload": f"CGFDM time loop {NX}x{NY}x{NZ}, dh 100 m, DT 5 ms, random terrain "
+                                   f"1 km RMS, layered medium + 5% random, double-couple table source, "
+                                   f"21x21 stations, PGV, nPML 12, free surface",
+                       "grid": [NX, NY, NZ], "storage": head, "arith": "f32"},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
+                         "traffic": tr["bytes_per_launch"] if tr else None,
+                         "kernel": "k_stage_bulk (fused RK stage over the bulk box)",
+                         "bytes_per_launch": bulk_bytes, "launch_ms": round(r["bulk_ms"], 4),
+                         "traffic_source": tr["source"] if tr else None,
+                         "whole_stage": {"ms": round(r["stage_avg_ms"], 4),
+                                         "achieved": round(VALUES_PER_LUP / 4 * sb * NX * NY * NZ
+                                                           / (r["stage_avg_ms"] * 1e-3) / 1e9, 1)}},
+        })
+    out["cpu_baseline"] = cpu
     for prec, v in res.items():
-        out[f"{prec}_store"] = {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 3),
-                                "stage_avg_ms": round(v["stage_avg_ms"], 4), "finite": v["finite"]}
+        out[f"{prec}_store"] = {k: (round(x, 4) if isinstance(x, float) else x) for k, x in v.items()}
     if "fp16" in res and "fp32" in res:
         out["fp16_over_fp32"] = round(res["fp16"]["glups"] / res["fp32"]["glups"], 3)
-    out["latency_p50_ms"] = None
-    out["latency_p99_ms"] = None
+    if serve:
+        out["latency_p50_ms"] = round(serve["batched"]["p50_s"] * 1e3, 1)
+        out["latency_p99_ms"] = round(serve["batched"]["p99_s"] * 1e3, 1)
+        out["serving"] = serve
+    else:
+        out["latency_p50_ms"] = None
+        out["latency_p99_ms"] = None
     print(json.dumps(out), flush=True)
 
 

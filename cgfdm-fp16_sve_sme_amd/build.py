@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcgfdm.so")
 CLI = os.path.join(HERE, "cgfdm_run")
-SOURCES = ["setup.hip", "stage.hip", "misc.hip", "api.hip"]
+SOURCES = ["setup.hip", "stage.hip", "misc.hip", "api.hip", "batch.hip", "server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 on the device (the host pass only warns about it).

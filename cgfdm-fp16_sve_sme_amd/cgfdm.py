@@ -88,7 +88,16 @@ def lib():
             getattr(L, n).argtypes = [vp, vp]
         L.cgfdm_get_slice.argtypes = [vp, i, i, i, vp]
         L.cgfdm_stage_timing.argtypes = [vp, i, P(ctypes.c_double), P(ll)]
+        L.cgfdm_kernel_timing.argtypes = [vp, i, vp, P(ll)]
         L.cgfdm_enable_timing.argtypes = [vp, i]
+        L.cgfdm_partition.argtypes = [vp, vp]
+        L.cgfdm_set_model_layered.argtypes = [vp, vp, i, vp, vp, vp, vp]
+        L.cgfdm_server_create.argtypes = [vp, P(vp)]
+        L.cgfdm_server_submit.argtypes = [vp, vp, P(ll)]
+        L.cgfdm_server_wait.argtypes = [vp, vp, i, i]
+        L.cgfdm_server_result.argtypes = [vp, ll, vp, vp, vp]
+        L.cgfdm_server_stats_get.argtypes = [vp, vp]
+        L.cgfdm_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -203,6 +212,24 @@ class Domain:
     def enable_timing(self, on=True):
         _chk(lib().cgfdm_enable_timing(self.h, 1 if on else 0))
 
+    def set_model_layered(self, terrain, depth, vs, vp, rho):
+        t = None if terrain is None else np.ascontiguousarray(terrain, dtype=np.float64)
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (depth, vs, vp, rho)]
+        _chk(lib().cgfdm_set_model_layered(self.h, _p(t), len(a[0]), *[_p(x) for x in a]))
+
+    def kernel_timing(self, reset=False):
+        """(stage, bulk kernel, boundary kernel) average ms per stage, and the stages timed."""
+        out = np.zeros(3, np.float64)
+        cnt = ctypes.c_longlong()
+        _chk(lib().cgfdm_kernel_timing(self.h, 1 if reset else 0, _p(out), ctypes.byref(cnt)))
+        return tuple(out), cnt.value
+
+    def partition(self):
+        """(bulk points, boundary points, tile shape, z-chunk) of a stage."""
+        out = np.zeros(4, np.int64)
+        _chk(lib().cgfdm_partition(self.h, _p(out)))
+        return tuple(int(x) for x in out)
+
     def stage_timing(self, reset=False):
         avg = ctypes.c_double()
         cnt = ctypes.c_longlong()
@@ -212,3 +239,142 @@ class Domain:
 
 def estimate_bytes(desc):
     return lib().cgfdm_estimate_bytes(ctypes.byref(desc))
+
+
+# ------------------------------------------------------------------ serving
+class ServerConfig(ctypes.Structure):
+    """Mirror of struct cgfdm_server_config."""
+    _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
+                ("hbm_budget", ctypes.c_longlong), ("host_budget", ctypes.c_longlong),
+                ("max_domains", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """Mirror of struct cgfdm_request."""
+    _fields_ = [("desc", Desc), ("terrain", ctypes.c_void_p), ("nlayer", ctypes.c_int),
+                ("layer_depth", ctypes.c_void_p), ("layer_vs", ctypes.c_void_p),
+                ("layer_vp", ctypes.c_void_p), ("layer_rho", ctypes.c_void_p),
+                ("vs", ctypes.c_void_p), ("vp", ctypes.c_void_p), ("rho", ctypes.c_void_p),
+                ("source", ctypes.c_int), ("src_xyz", ctypes.c_int * 3), ("fc", ctypes.c_float),
+                ("tab_xyz", ctypes.c_void_p), ("tab_rate", ctypes.c_void_p),
+                ("tab_npts", ctypes.c_longlong), ("tab_nt", ctypes.c_int), ("tab_dt", ctypes.c_float),
+                ("st_xyz", ctypes.c_void_p), ("nst", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    """Mirror of struct cgfdm_request_info."""
+    _fields_ = [("status", ctypes.c_int), ("NT", ctypes.c_int), ("nst", ctypes.c_int),
+                ("NX", ctypes.c_int), ("NY", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("device_bytes", ctypes.c_longlong), ("error", ctypes.c_char * 160)]
+
+
+class ServerStats(ctypes.Structure):
+    """Mirror of struct cgfdm_server_stats."""
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("failed", ctypes.c_longlong), ("launches", ctypes.c_longlong),
+                ("resident", ctypes.c_int), ("pending", ctypes.c_int), ("peak_resident", ctypes.c_int),
+                ("graph", ctypes.c_int), ("tile_cfg", ctypes.c_int), ("bulk_blocks", ctypes.c_int),
+                ("general_blocks", ctypes.c_int),
+                ("arena_bytes", ctypes.c_longlong), ("arena_used", ctypes.c_longlong),
+                ("arena_peak", ctypes.c_longlong)]
+
+
+class Server:
+    """Continuous-batching server (cgfdm_server): submit requests, wait for completions, fetch
+    results. Requests are dicts with keys desc, and either layers=(depth, vs, vp, rho) or
+    medium=(vs, vp, rho); terrain (with desc.use_terrain); point=(X, Y, Z, fc) or
+    table=(xyz, rate[nt][npts][6], dt); stations (n x 3)."""
+
+    def __init__(self, precision=FP16, device=0, hbm_budget=0, host_budget=0, max_domains=16,
+                 steps_per_launch=8, use_graph=True):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        c = ServerConfig(device, precision, hbm_budget, host_budget, max_domains, steps_per_launch,
+                         1 if use_graph else 0)
+        h = ctypes.c_void_p()
+        _chk(lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+        self.meta = {}
+
+    def submit(self, req):
+        r = Request()
+        r.desc = req["desc"]
+        keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        if req.get("terrain") is not None:
+            r.terrain = arr(req["terrain"], np.float64)
+        if "layers" in req:
+            dep, vs, vp, rho = req["layers"]
+            r.nlayer = len(dep)
+            r.layer_depth, r.layer_vs = arr(dep, np.float32), arr(vs, np.float32)
+            r.layer_vp, r.layer_rho = arr(vp, np.float32), arr(rho, np.float32)
+        else:
+            vs, vp, rho = req["medium"]
+            r.vs, r.vp, r.rho = arr(vs, np.float32), arr(vp, np.float32), arr(rho, np.float32)
+        if "point" in req:
+            X, Y, Z, fc = req["point"]
+            r.source = 1
+            r.src_xyz[0], r.src_xyz[1], r.src_xyz[2] = X, Y, Z
+            r.fc = fc
+        else:
+            xyz, rate, dt = req["table"]
+            rate = np.asarray(rate, np.float32)
+            r.source = 2
+            r.tab_xyz, r.tab_rate = arr(xyz, np.int32), arr(rate, np.float32)
+            r.tab_npts, r.tab_nt, r.tab_dt = rate.shape[1], rate.shape[0], dt
+        st = np.ascontiguousarray(req.get("stations", np.zeros((0, 3))), dtype=np.int32)
+        r.st_xyz = arr(st, np.int32)
+        r.nst = st.shape[0]
+        rid = ctypes.c_longlong()
+        _chk(lib().cgfdm_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        d = req["desc"]
+        self.meta[rid.value] = (r.nst, d.NT, d.NX, d.NY)
+        return rid.value
+
+    def wait(self, max_ids=64, timeout_ms=-1):
+        ids = np.zeros(max_ids, np.int64)
+        n = lib().cgfdm_server_wait(self.h, _p(ids), max_ids, timeout_ms)
+        if n < 0:
+            _chk(n)
+        return [int(x) for x in ids[:n]]
+
+    def result(self, rid):
+        """(status, stations, pgv, info dict) of a completed request; the record is released."""
+        nst, NT, NX, NY = self.meta.pop(rid)
+        st = np.zeros((nst, NT, 9), np.float32)
+        pgv = np.zeros((NY, NX, 2), np.float32)
+        info = RequestInfo()
+        rc = lib().cgfdm_server_result(self.h, rid, _p(st), _p(pgv), ctypes.byref(info))
+        i = {"status": info.status, "t_submit": info.t_submit, "t_admit": info.t_admit,
+             "t_done": info.t_done, "device_bytes": info.device_bytes,
+             "error": info.error.decode(errors="replace")}
+        return rc, st, pgv, i
+
+    def stats(self):
+        s = ServerStats()
+        _chk(lib().cgfdm_server_stats_get(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().cgfdm_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,26 +1,29 @@
 // api.hip — host implementation of the C ABI declared in include/cgfdm.h.
 //
-// One domain = one simulation on one GPU with its own HIP stream (default priority).
-// Device memory layout: every 3-D array is k-major, [k][var][j][i] (see Layout in common.hpp):
+// One domain = one simulation on one GPU. Device memory layout: every 3-D array is k-major,
+// [k][var][j][i] (Layout in common.hpp):
 //   W buffers A, B, C and accumulator T : 9 variables, storage type
 //   CJM                                  : 13 components, storage type
 //   PML slabs                            : 3 registers x 9 x slab points, FP32
 //   rdz, profiles, stations, PGV         : FP32
-#include <cstdarg>
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
-#include "../../include/cgfdm.h"
-#include "common.hpp"
+#include "domain.hpp"
 
 using namespace cg;
 
-namespace {
+namespace cg {
 
 thread_local std::string g_err;
 
@@ -32,6 +35,8 @@ void set_err(const char* fmt, ...) {
     va_end(ap);
     g_err = buf;
 }
+
+}  // namespace cg
 
 #define HIPCHK(x)                                                                   \
     do {                                                                            \
@@ -51,75 +56,64 @@ void set_err(const char* fmt, ...) {
         }                                                                           \
     } while (0)
 
+namespace {
+
 const int FB_TABLE[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
                             {-1, 1, -1},  {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};  // propagate.cpp:107-117
 
 }  // namespace
 
-struct cgfdm_domain {
-    cgfdm_desc d{};
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    int prec = P_F32;
-    int sb = 4;  // storage bytes
-    int nx = 0, ny = 0, nz = 0, nx_ = 0, ny_ = 0, nz_ = 0;
-    Layout lay{};
-    double Cv = 1, Cs = 1, Crho = 1;
-    void* W[4] = {nullptr, nullptr, nullptr, nullptr};  // A, B, C, T
-    void* cjm = nullptr;
-    float* prof = nullptr;  // [ax bx dx][ay by dy][az bz dz]
-    float* rdz = nullptr;
-    float* auxmem = nullptr;
-    Slab slab[6]{};
-    float* pgv = nullptr;
-    // stations (padded i, j, k triples)
-    int* stIdx = nullptr;
-    float* stWave = nullptr;
-    int nst = 0;
-    // sources
-    int srcMode = 0;
-    int SX = 0, SY = 0, SZ = 0;
-    float fc = 0.f;
-    float* srcCoef = nullptr;
-    int* srcIdx = nullptr;  // padded i, j, k triples
-    float* mrate = nullptr;
-    long long npts = 0;
-    int snt = 0;
-    float sdt = 0.f;
-    bool multiActive = false;
-    int it = 0;
-    bool modelSet = false;
-    int bulk_cfg = 0, bulk_kc = 0;  // launch shape of the bulk stage kernel
-    long long bytes = 0;
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> evStart, evStop;
-    size_t evUsed = 0;
-    double tAccum = 0.0;
-    long long tCount = 0;
-    std::vector<void*> allocs;
-
-    int alloc(void** p, size_t n) {
-        if (n == 0) { *p = nullptr; return 0; }
+// ------------------------------------------------------------------ domain memory
+int cgfdm_domain::alloc(void** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return 0;
+    if (pooled) {
+        *p = devPool.take(n);
+        if (!*p) {
+            set_err("domain block exhausted (%zu bytes more requested, %zu of %zu used)", n, devPool.used, devPool.size);
+            return CGFDM_ENOMEM;
+        }
+    } else {
         hipError_t e = hipMalloc(p, n);
         if (e != hipSuccess) {
             (void)hipGetLastError();
+            *p = nullptr;
             set_err("hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
             return CGFDM_ENOMEM;
         }
         allocs.push_back(*p);
-        bytes += (long long)n;
-        e = hipMemsetAsync(*p, 0, n, stream);
-        if (e != hipSuccess) { set_err("hipMemsetAsync failed: %s", hipGetErrorString(e)); return CGFDM_EHIP; }
-        return 0;
     }
-    void release(void* p) {
-        if (!p) return;
-        for (size_t q = 0; q < allocs.size(); ++q)
-            if (allocs[q] == p) { allocs.erase(allocs.begin() + q); break; }
-        (void)hipFree(p);
+    bytes += (long long)n;
+    hipError_t e = hipMemsetAsync(*p, 0, n, stream);
+    if (e != hipSuccess) { set_err("hipMemsetAsync failed: %s", hipGetErrorString(e)); return CGFDM_EHIP; }
+    return 0;
+}
+
+void cgfdm_domain::release(void* p) {
+    if (!p || pooled) return;
+    for (size_t q = 0; q < allocs.size(); ++q)
+        if (allocs[q] == p) { allocs.erase(allocs.begin() + q); break; }
+    (void)hipFree(p);
+}
+
+int cgfdm_domain::h2d(void* dst, const void* src, size_t n) {
+    if (n == 0) return 0;
+    const void* from = src;
+    if (pooled) {
+        void* st = hostPool.take(n);
+        if (!st) { set_err("pinned staging block exhausted (%zu bytes)", n); return CGFDM_ENOMEM; }
+        std::memcpy(st, src, n);
+        from = st;
     }
-};
+    HIPCHK(hipMemcpyAsync(dst, from, n, hipMemcpyHostToDevice, stream));
+    return 0;
+}
+
+int cgfdm_domain::settle() {
+    if (async) return 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    return 0;
+}
 
 namespace {
 
@@ -228,22 +222,244 @@ SetupParams setup_params(const cgfdm_domain* h, const double* dterrain) {
     return q;
 }
 
-float ricker_moment(float rickerfc, int it, float DT) {
-    // sourceFunction at irk = 0 (singleSource.cpp:29-56), M0 = 1e16
-    float t = ((float)it + 0.0f) * DT;
-    float tdelay = 1.2f / rickerfc;
-    float f0 = 2;
-    float r = PIf * rickerfc * (t - tdelay);
-    float rr = r * r;
-    float s = (float)(r * (2.0f * rr - 3.0) * std::exp(-rr) * f0 * PIf * rickerfc);
-    return s * 1e16f;
-}
-
 void* stage_in(cgfdm_domain* h, int irk) { return h->W[irk == 0 ? 0 : (irk == 2 ? 2 : 1)]; }
 void* stage_out(cgfdm_domain* h, int irk) { return h->W[irk == 0 ? 1 : (irk == 1 ? 2 : (irk == 2 ? 1 : 0))]; }
 
+// ------------------------------------------------------------------ bulk launch-shape tuner
+// Candidate shapes of the bulk kernel: tile (64x8 or 64x4 columns) and z-chunk length (0 picks
+// enough chunks for ~2048 workgroups). The tuner times each on stage 0 of the domain's first
+// step, which only reads A and the metric and only writes B and T, so repeating it is harmless;
+// the choice is cached per (precision, grid, nPML, free surface) for the process.
+struct TuneKey {
+    int prec, nx, ny, nz, npml, fs;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(prec, nx, ny, nz, npml, fs) < std::tie(o.prec, o.nx, o.ny, o.nz, o.npml, o.fs);
+    }
+};
+std::mutex g_tuneMu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+const int kTuneCands[][2] = {{0, 0}, {0, 64}, {1, 0}, {1, 32}, {1, 64}};
+
+int tune_bulk(cgfdm_domain* h, StageParams p) {
+    const TuneKey key{h->prec, h->nx, h->ny, h->nz, h->d.nPML, h->d.free_surface};
+    {
+        std::lock_guard<std::mutex> g(g_tuneMu);
+        auto f = g_tuned.find(key);
+        if (f != g_tuned.end()) {
+            h->bulk_cfg = f->second.first;
+            h->bulk_kc = f->second.second;
+            return 0;
+        }
+    }
+    p.Win = stage_in(h, 0);
+    p.Wout = stage_out(h, 0);
+    p.irk = 0;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bc = 1, bk = 0;
+    for (auto& c : kTuneCands) {
+        p.bulk_cfg = c[0];
+        p.bulk_kc = c[1];
+        LCHK(launch_stage(h->prec, p, 2, h->stream, nullptr));  // warm-up
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int r = 0; r < 2; ++r) LCHK(launch_stage(h->prec, p, 2, h->stream, nullptr));
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; bc = c[0]; bk = c[1]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->bulk_cfg = bc;
+    h->bulk_kc = bk;
+    std::lock_guard<std::mutex> g(g_tuneMu);
+    g_tuned[key] = {bc, bk};
+    return 0;
+}
+
 }  // namespace
 
+// ------------------------------------------------------------------ shared set-up
+namespace cg {
+
+int validate_desc(const cgfdm_desc* d) { return validate(d); }
+long long domain_estimate(const cgfdm_desc& d) { return estimate(d); }
+
+int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
+    h->d = *desc;
+    h->dev = device;
+    h->prec = desc->precision;
+    h->sb = storage_bytes(h->prec);
+    h->nx = desc->NX; h->ny = desc->NY; h->nz = desc->NZ;
+    h->nx_ = h->nx + 6; h->ny_ = h->ny + 6; h->nz_ = h->nz + 6;
+    h->lay = make_layout(*desc);
+    const bool low = h->prec != P_F32;
+    h->Cv = desc->Cv > 0 ? desc->Cv : (low ? 5.0e3 : 1.0);
+    h->Cs = desc->Cs > 0 ? desc->Cs : (low ? 1.0e-6 : 1.0);
+    h->Crho = desc->Crho > 0 ? desc->Crho : (low ? 1.0e6 : 1.0);
+    if (const char* e = std::getenv("CGFDM_BULK_CFG")) h->bulk_cfg = std::atoi(e);
+    if (const char* e = std::getenv("CGFDM_BULK_KC")) h->bulk_kc = std::atoi(e);
+    int rc;
+    for (int b = 0; b < 4; ++b)
+        if ((rc = h->alloc(&h->W[b], (size_t)h->lay.size(9) * h->sb))) return rc;
+    if ((rc = h->alloc(&h->cjm, (size_t)h->lay.size(13) * h->sb))) return rc;
+    // PML slabs: x1 y1 z1 x2 y2 z2
+    long long auxTotal = 0;
+    for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
+    if (auxTotal > 0 && (rc = h->alloc((void**)&h->auxmem, (size_t)auxTotal * 9 * 3 * sizeof(float)))) return rc;
+    long long off = 0;
+    for (int s = 0; s < 6; ++s) {
+        Slab& X = h->slab[s];
+        X.n = slab_points(h->d, s);
+        X.on = X.n > 0;
+        X.axis = s % 3;
+        const int nAxis = X.axis == 0 ? h->nx : (X.axis == 1 ? h->ny : h->nz);
+        X.lo = s < 3 ? HALO : HALO + nAxis - h->d.nPML;
+        X.n0 = X.axis == 0 ? h->d.nPML : h->nx;
+        X.n1 = X.axis == 1 ? h->d.nPML : h->ny;
+        if (X.on) {
+            X.a = h->auxmem + off;
+            X.m = X.a + 9 * X.n;
+            X.t = X.m + 9 * X.n;
+            off += 27 * X.n;
+        }
+    }
+    if ((rc = h->alloc((void**)&h->prof, sizeof(float) * 3 * (h->nx_ + h->ny_ + h->nz_)))) return rc;
+    if ((rc = h->alloc((void**)&h->rdz, sizeof(float) * 18 * (size_t)h->nx_ * h->ny_))) return rc;
+    if ((rc = h->alloc((void**)&h->pgv, sizeof(float) * 2 * (size_t)h->nx * h->ny))) return rc;
+    std::vector<float> prof;
+    pml_profiles(h->d, prof);
+    if ((rc = h->h2d(h->prof, prof.data(), prof.size() * sizeof(float)))) return rc;
+    return h->settle();
+}
+
+int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& host) {
+    if (h->d.use_terrain && !terrain) { set_err("use_terrain set but no terrain given"); return CGFDM_EINVAL; }
+    HIPCHK(hipSetDevice(h->dev));
+    const long long nin = (long long)h->nx * h->ny * h->nz;
+    MediumSrc dev{};
+    std::vector<void*> temps;
+    auto cleanup = [&]() { for (void* p : temps) h->release(p); };
+    auto temp = [&](void** p, size_t n) {
+        int rc = h->alloc(p, n);
+        if (rc == 0) { temps.push_back(*p); h->bytes -= (long long)n; }
+        return rc;
+    };
+    int rc = 0;
+    float vpmax_host = 0.f;
+    if (host.vs) {
+        for (long long q = 0; q < nin; ++q) {
+            if (!(host.vs[q] >= 0.f) || !(host.vp[q] > 0.f) || !(host.rho[q] > 0.f)) {
+                set_err("medium must be finite with vs >= 0, vp, rho > 0 (index %lld)", q);
+                return CGFDM_EINVAL;
+            }
+            vpmax_host = std::max(vpmax_host, host.vp[q]);
+        }
+        const size_t mb = sizeof(float) * (size_t)nin;
+        float *a = nullptr, *b = nullptr, *c = nullptr;
+        if ((rc = temp((void**)&a, mb)) || (rc = temp((void**)&b, mb)) || (rc = temp((void**)&c, mb))) {
+            cleanup();
+            return rc;
+        }
+        if ((rc = h->h2d(a, host.vs, mb)) || (rc = h->h2d(b, host.vp, mb)) || (rc = h->h2d(c, host.rho, mb))) {
+            cleanup();
+            return rc;
+        }
+        dev.vs = a; dev.vp = b; dev.rho = c;
+    } else {
+        if (host.nl < 1 || !host.layers) { set_err("layered medium needs at least one layer"); return CGFDM_EINVAL; }
+        for (int l = 0; l < host.nl; ++l) {
+            const float* L = host.layers;
+            if (!(L[host.nl + l] >= 0.f) || !(L[2 * host.nl + l] > 0.f) || !(L[3 * host.nl + l] > 0.f) ||
+                (l > 0 && !(L[l] >= L[l - 1]))) {
+                set_err("layer %d: depths must increase and vs >= 0, vp, rho > 0", l);
+                return CGFDM_EINVAL;
+            }
+            vpmax_host = std::max(vpmax_host, L[2 * host.nl + l]);
+        }
+        vpmax_host = std::max(vpmax_host, 6000.0f);  // points above the first interface keep the default
+        float* t = nullptr;
+        const size_t n = sizeof(float) * 4 * host.nl;
+        if ((rc = temp((void**)&t, n)) || (rc = h->h2d(t, host.layers, n))) { cleanup(); return rc; }
+        dev.layers = t;
+        dev.nl = host.nl;
+    }
+    double* dter = nullptr;
+    if (h->d.use_terrain) {
+        const size_t tb = sizeof(double) * (size_t)h->nx_ * h->ny_;
+        if ((rc = temp((void**)&dter, tb)) || (rc = h->h2d(dter, terrain, tb))) { cleanup(); return rc; }
+    }
+    SetupParams q = setup_params(h, dter);
+    LCHK(launch_setup_medium(h->prec, q, dev, h->cjm, h->stream));
+    LCHK(launch_setup_metric(h->prec, q, h->cjm, h->stream));
+    if (h->d.free_surface) LCHK(launch_setup_rdz(q, dev, h->rdz, h->stream));
+    if (h->d.check_cfl) {
+        if (h->async) { cleanup(); set_err("the CFL check needs a synchronous domain"); return CGFDM_EINVAL; }
+        // calc_CFL.cpp:331-352: dtmax = 1.34*Hmin/Vpmax
+        unsigned long long* dmin = nullptr;
+        unsigned* dvp = nullptr;
+        if ((rc = temp((void**)&dmin, sizeof(unsigned long long))) || (rc = temp((void**)&dvp, sizeof(unsigned)))) {
+            cleanup();
+            return rc;
+        }
+        double init = 1.0e20;
+        HIPCHK(hipMemcpyAsync(dmin, &init, 8, hipMemcpyHostToDevice, h->stream));
+        LCHK(launch_hmin(q, dmin, h->stream));
+        LCHK(launch_vpmax(q, dev, dvp, h->stream));
+        double hmin = 0;
+        unsigned vbits = 0;
+        HIPCHK(hipMemcpyAsync(&hmin, dmin, 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(&vbits, dvp, 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        float vpmax = 0.f;
+        std::memcpy(&vpmax, &vbits, 4);
+        const double dtmax = 1.34 * hmin / vpmax;
+        if (h->d.DT > dtmax) {
+            cleanup();
+            set_err("DT = %.4e exceeds the CFL limit dtmax = %.4e (Hmin %.4e, Vpmax %.4e)", h->d.DT, dtmax, hmin, vpmax);
+            return CGFDM_ECFL;
+        }
+    }
+    if ((rc = h->settle())) return rc;
+    cleanup();
+    h->modelSet = true;
+    return CGFDM_OK;
+}
+
+StageParams domain_stage_params(const cgfdm_domain* h) {
+    StageParams p{};
+    p.A = h->W[0];
+    p.T = h->W[3];
+    p.cjm = h->cjm;
+    const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
+    const int lens[3] = {h->nx_, h->ny_, h->nz_};
+    for (int a = 0; a < 3; ++a) {
+        p.pa[a] = h->prof + offs[a];
+        p.pb[a] = h->prof + offs[a] + lens[a];
+        p.pd[a] = h->prof + offs[a] + 2 * lens[a];
+    }
+    p.rdz = h->rdz;
+    for (int s = 0; s < 6; ++s) p.slab[s] = h->slab[s];
+    p.lay = h->lay;
+    p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
+    p.nx_ = h->nx_; p.ny_ = h->ny_;
+    p.nPML = h->d.nPML;
+    p.fs = h->d.free_surface;
+    p.pml = h->d.pml;
+    p.bulk_cfg = h->bulk_cfg < 0 ? 1 : h->bulk_cfg;
+    p.bulk_kc = h->bulk_kc;
+    p.rDH = (float)(1.0 / h->d.DH);
+    p.DT = (float)h->d.DT;
+    p.Crho = (float)h->Crho;
+    return p;
+}
+
+}  // namespace cg
+
+// ------------------------------------------------------------------ C ABI
 extern "C" {
 
 const char* cgfdm_last_error(void) { return g_err.c_str(); }
@@ -263,57 +479,12 @@ int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) 
     if (device < 0 || device >= ndev) { set_err("device %d not present (%d devices)", device, ndev); return CGFDM_EINVAL; }
     HIPCHK(hipSetDevice(device));
     cgfdm_domain* h = new cgfdm_domain();
-    h->d = *desc;
-    h->dev = device;
-    h->prec = desc->precision;
-    h->sb = storage_bytes(h->prec);
-    h->nx = desc->NX; h->ny = desc->NY; h->nz = desc->NZ;
-    h->nx_ = h->nx + 6; h->ny_ = h->ny + 6; h->nz_ = h->nz + 6;
-    h->lay = make_layout(*desc);
-    const bool low = h->prec != P_F32;
-    h->Cv = desc->Cv > 0 ? desc->Cv : (low ? 5.0e3 : 1.0);
-    h->Cs = desc->Cs > 0 ? desc->Cs : (low ? 1.0e-6 : 1.0);
-    h->Crho = desc->Crho > 0 ? desc->Crho : (low ? 1.0e6 : 1.0);
-    if (const char* e = std::getenv("CGFDM_BULK_CFG")) h->bulk_cfg = std::atoi(e);
-    if (const char* e = std::getenv("CGFDM_BULK_KC")) h->bulk_kc = std::atoi(e);
     hipError_t e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) { set_err("hipStreamCreate: %s", hipGetErrorString(e)); delete h; return CGFDM_EHIP; }
-    auto fail = [&](int code) { cgfdm_domain_destroy(h); return code; };
-    for (int b = 0; b < 4; ++b)
-        if ((rc = h->alloc(&h->W[b], (size_t)h->lay.size(9) * h->sb))) return fail(rc);
-    if ((rc = h->alloc(&h->cjm, (size_t)h->lay.size(13) * h->sb))) return fail(rc);
-    // PML slabs: x1 y1 z1 x2 y2 z2
-    long long auxTotal = 0;
-    for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
-    if (auxTotal > 0) {
-        if ((rc = h->alloc((void**)&h->auxmem, (size_t)auxTotal * 9 * 3 * sizeof(float)))) return fail(rc);
+    if ((rc = domain_init(h, desc, device))) {
+        cgfdm_domain_destroy(h);
+        return rc;
     }
-    long long off = 0;
-    for (int s = 0; s < 6; ++s) {
-        Slab& X = h->slab[s];
-        X.n = slab_points(h->d, s);
-        X.on = X.n > 0;
-        X.axis = s % 3;
-        const int nAxis = X.axis == 0 ? h->nx : (X.axis == 1 ? h->ny : h->nz);
-        X.lo = s < 3 ? HALO : HALO + nAxis - h->d.nPML;
-        X.n0 = X.axis == 0 ? h->d.nPML : h->nx;
-        X.n1 = X.axis == 1 ? h->d.nPML : h->ny;
-        if (X.on) {
-            X.a = h->auxmem + off;
-            X.m = X.a + 9 * X.n;
-            X.t = X.m + 9 * X.n;
-            off += 27 * X.n;
-        }
-    }
-    if ((rc = h->alloc((void**)&h->prof, sizeof(float) * 3 * (h->nx_ + h->ny_ + h->nz_)))) return fail(rc);
-    if ((rc = h->alloc((void**)&h->rdz, sizeof(float) * 18 * (size_t)h->nx_ * h->ny_))) return fail(rc);
-    if ((rc = h->alloc((void**)&h->pgv, sizeof(float) * 2 * (size_t)h->nx * h->ny))) return fail(rc);
-    std::vector<float> prof;
-    pml_profiles(h->d, prof);
-    e = hipMemcpyAsync(h->prof, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice, h->stream);
-    if (e != hipSuccess) { set_err("profile upload: %s", hipGetErrorString(e)); return fail(CGFDM_EHIP); }
-    e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) { set_err("create sync: %s", hipGetErrorString(e)); return fail(CGFDM_EHIP); }
     *out = h;
     return CGFDM_OK;
 }
@@ -325,7 +496,7 @@ int cgfdm_domain_destroy(cgfdm_domain* h) {
     for (void* p : h->allocs) (void)hipFree(p);
     for (auto e : h->evStart) (void)hipEventDestroy(e);
     for (auto e : h->evStop) (void)hipEventDestroy(e);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->ownStream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CGFDM_OK;
 }
@@ -334,65 +505,22 @@ long long cgfdm_domain_bytes(const cgfdm_domain* h) { return h ? h->bytes : 0; }
 
 int cgfdm_set_model(cgfdm_domain* h, const double* terrain, const float* vs, const float* vp, const float* rho) {
     if (!h || !vs || !vp || !rho) { set_err("null argument"); return CGFDM_EINVAL; }
-    if (h->d.use_terrain && !terrain) { set_err("use_terrain set but no terrain given"); return CGFDM_EINVAL; }
-    HIPCHK(hipSetDevice(h->dev));
-    const long long nin = (long long)h->nx * h->ny * h->nz;
-    float vpmax = 0.f;
-    for (long long q = 0; q < nin; ++q) {
-        if (!(vs[q] >= 0.f) || !(vp[q] > 0.f) || !(rho[q] > 0.f)) {
-            set_err("medium must be finite with vp, rho > 0 (index %lld)", q);
-            return CGFDM_EINVAL;
-        }
-        vpmax = std::max(vpmax, vp[q]);
+    MediumSrc m{vs, vp, rho, nullptr, 0};
+    return domain_set_model(h, terrain, m);
+}
+
+int cgfdm_set_model_layered(cgfdm_domain* h, const double* terrain, int nlayer, const float* depth, const float* vs,
+                            const float* vp, const float* rho) {
+    if (!h || nlayer < 1 || !depth || !vs || !vp || !rho) { set_err("bad layered model"); return CGFDM_EINVAL; }
+    std::vector<float> L(4 * (size_t)nlayer);
+    for (int l = 0; l < nlayer; ++l) {
+        L[l] = depth[l];
+        L[nlayer + l] = vs[l];
+        L[2 * nlayer + l] = vp[l];
+        L[3 * nlayer + l] = rho[l];
     }
-    float *dvs = nullptr, *dvp = nullptr, *drho = nullptr;
-    double* dter = nullptr;
-    int rc = 0;
-    const size_t mb = sizeof(float) * (size_t)nin;
-    if ((rc = h->alloc((void**)&dvs, mb)) || (rc = h->alloc((void**)&dvp, mb)) || (rc = h->alloc((void**)&drho, mb))) {
-        h->release(dvs); h->release(dvp); h->release(drho);
-        return rc;
-    }
-    h->bytes -= 3 * (long long)mb;
-    auto cleanup = [&]() {
-        h->release(dvs); h->release(dvp); h->release(drho); h->release(dter);
-    };
-    if (h->d.use_terrain) {
-        const size_t tb = sizeof(double) * (size_t)h->nx_ * h->ny_;
-        if ((rc = h->alloc((void**)&dter, tb))) { cleanup(); return rc; }
-        h->bytes -= (long long)tb;
-        HIPCHK(hipMemcpyAsync(dter, terrain, tb, hipMemcpyHostToDevice, h->stream));
-    }
-    HIPCHK(hipMemcpyAsync(dvs, vs, mb, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(dvp, vp, mb, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(drho, rho, mb, hipMemcpyHostToDevice, h->stream));
-    SetupParams q = setup_params(h, dter);
-    LCHK(launch_setup_medium(h->prec, q, dvs, dvp, drho, h->cjm, h->stream));
-    LCHK(launch_setup_metric(h->prec, q, h->cjm, h->stream));
-    if (h->d.free_surface) LCHK(launch_setup_rdz(q, dvs, dvp, drho, h->rdz, h->stream));
-    if (h->d.check_cfl) {
-        // calc_CFL.cpp:331-352: dtmax = 1.34*Hmin/Vpmax
-        unsigned long long* dmin = nullptr;
-        if ((rc = h->alloc((void**)&dmin, sizeof(unsigned long long)))) { cleanup(); return rc; }
-        h->bytes -= 8;
-        double init = 1.0e20;
-        HIPCHK(hipMemcpyAsync(dmin, &init, 8, hipMemcpyHostToDevice, h->stream));
-        LCHK(launch_hmin(q, dmin, h->stream));
-        double hmin = 0;
-        HIPCHK(hipMemcpyAsync(&hmin, dmin, 8, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        h->release(dmin);
-        const double dtmax = 1.34 * hmin / vpmax;
-        if (h->d.DT > dtmax) {
-            cleanup();
-            set_err("DT = %.4e exceeds the CFL limit dtmax = %.4e (Hmin %.4e, Vpmax %.4e)", h->d.DT, dtmax, hmin, vpmax);
-            return CGFDM_ECFL;
-        }
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    cleanup();
-    h->modelSet = true;
-    return CGFDM_OK;
+    MediumSrc m{nullptr, nullptr, nullptr, L.data(), nlayer};
+    return domain_set_model(h, terrain, m);
 }
 
 int cgfdm_set_source_point(cgfdm_domain* h, int X, int Y, int Z, float fc) {
@@ -404,38 +532,30 @@ int cgfdm_set_source_point(cgfdm_domain* h, int X, int Y, int Z, float fc) {
     }
     const int SX = X + HALO, SY = Y + HALO, SZ = Z + HALO;
     // the 7^3 smoothing stencil must stay inside the padded grid
-    if (SX < 3 || SY < 3 || SZ < 3 || SX + 3 >= h->nx_ || SY + 3 >= h->ny_ || SZ + 3 >= h->nz_) {
+    if (SX + 3 >= h->nx_ || SY + 3 >= h->ny_ || SZ + 3 >= h->nz_) {
         set_err("source too close to the grid edge");
         return CGFDM_EINVAL;
     }
     HIPCHK(hipSetDevice(h->dev));
-    // J at the source from the stored CJM, as load_smooth_source reads it
-    float* tmp = nullptr;
-    int rc = h->alloc((void**)&tmp, sizeof(float));
-    if (rc) return rc;
-    h->bytes -= 4;
-    const long long idx = h->lay.at(13, 9, SX, SY, SZ);
-    LCHK(launch_decode(h->prec, (const char*)h->cjm + idx * h->sb, tmp, 1, h->stream));
-    float jacb = 0;
-    HIPCHK(hipMemcpyAsync(&jacb, tmp, 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->release(tmp);
-    const float DH = (float)h->d.DH, DT = (float)h->d.DT;
-    std::vector<float> coef(343);
+    // normalised Gaussian weights (GAUSS_FUN, macro.h:39; singleSource.cpp:100-110), in double
+    std::vector<float> amp(343);
     const double ra = 3 * 0.5;
     for (int q = 0; q < 343; ++q) {
         const int di = q % 7 - 3, dj = (q / 7) % 7 - 3, dk = q / 49 - 3;
         auto G = [&](int t) { return (float)(std::exp(-((t / ra) * (t / ra))) / (ra * 1.772453850905516)); };
-        float amp = G(di) * G(dj) * G(dk);
-        amp = (float)(amp / 0.998125703461425);
-        float value = -1.0f * amp / (jacb * (DH * DH * DH));
-        coef[q] = (float)(value * (h->Cs * DT));
+        float a = G(di) * G(dj) * G(dk);
+        amp[q] = (float)(a / 0.998125703461425);
     }
-    if (!h->srcCoef) {
-        if ((rc = h->alloc((void**)&h->srcCoef, 343 * sizeof(float)))) return rc;
-    }
-    HIPCHK(hipMemcpyAsync(h->srcCoef, coef.data(), 343 * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    int rc;
+    float* damp = nullptr;
+    if (!h->srcCoef && (rc = h->alloc((void**)&h->srcCoef, 343 * sizeof(float)))) return rc;
+    if ((rc = h->alloc((void**)&damp, 343 * sizeof(float)))) return rc;
+    h->bytes -= 343 * sizeof(float);
+    if ((rc = h->h2d(damp, amp.data(), 343 * sizeof(float)))) return rc;
+    LCHK(launch_source_coef(h->prec, h->cjm, h->lay, SX, SY, SZ, damp, (float)h->d.DH,
+                            h->Cs * (double)(float)h->d.DT, h->srcCoef, h->stream));
+    if ((rc = h->settle())) return rc;
+    h->release(damp);
     h->SX = SX; h->SY = SY; h->SZ = SZ; h->fc = fc;
     h->srcMode = 1;
     return CGFDM_OK;
@@ -477,14 +597,13 @@ int cgfdm_set_source_table(cgfdm_domain* h, const int* xyz, const float* rate, l
     int rc;
     if ((rc = h->alloc((void**)&h->srcIdx, sizeof(int) * 3 * u))) return rc;
     if ((rc = h->alloc((void**)&h->mrate, sizeof(float) * mr.size()))) return rc;
-    HIPCHK(hipMemcpyAsync(h->srcIdx, sidx.data(), sizeof(int) * 3 * u, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->mrate, mr.data(), sizeof(float) * mr.size(), hipMemcpyHostToDevice, h->stream));
+    if ((rc = h->h2d(h->srcIdx, sidx.data(), sizeof(int) * 3 * u))) return rc;
+    if ((rc = h->h2d(h->mrate, mr.data(), sizeof(float) * mr.size()))) return rc;
     LCHK(launch_scale_mrate(h->prec, h->cjm, h->lay, h->srcIdx, h->mrate, u, nt, (float)h->d.DH, (float)h->Cv,
                             h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if ((rc = h->settle())) return rc;
     h->npts = u; h->snt = nt; h->sdt = dt;
     h->srcMode = 2;
-    h->multiActive = true;
     return CGFDM_OK;
 }
 
@@ -507,57 +626,31 @@ int cgfdm_set_stations(cgfdm_domain* h, const int* xyz, int n) {
     int rc;
     if ((rc = h->alloc((void**)&h->stIdx, sizeof(int) * 3 * n))) return rc;
     if ((rc = h->alloc((void**)&h->stWave, sizeof(float) * (size_t)n * h->d.NT * 9))) return rc;
-    HIPCHK(hipMemcpyAsync(h->stIdx, idx.data(), sizeof(int) * 3 * n, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return CGFDM_OK;
+    if ((rc = h->h2d(h->stIdx, idx.data(), sizeof(int) * 3 * n))) return rc;
+    return h->settle();
 }
 
 int cgfdm_step(cgfdm_domain* h, int nsteps) {
     if (!h || nsteps < 0) { set_err("bad arguments"); return CGFDM_EINVAL; }
     if (!h->modelSet) { set_err("set_model must precede step"); return CGFDM_ESTATE; }
+    if (h->pooled) { set_err("a served domain is stepped by its server"); return CGFDM_ESTATE; }
     HIPCHK(hipSetDevice(h->dev));
     const float DT = (float)h->d.DT;
-    StageParams p{};
-    p.A = h->W[0];
-    p.T = h->W[3];
-    p.cjm = h->cjm;
-    const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
-    const int lens[3] = {h->nx_, h->ny_, h->nz_};
-    for (int a = 0; a < 3; ++a) {
-        p.pa[a] = h->prof + offs[a];
-        p.pb[a] = h->prof + offs[a] + lens[a];
-        p.pd[a] = h->prof + offs[a] + 2 * lens[a];
-    }
-    p.rdz = h->rdz;
-    for (int s = 0; s < 6; ++s) p.slab[s] = h->slab[s];
-    p.lay = h->lay;
-    p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
-    p.nx_ = h->nx_; p.ny_ = h->ny_;
-    p.nPML = h->d.nPML;
-    p.fs = h->d.free_surface;
-    p.pml = h->d.pml;
-    p.bulk_cfg = h->bulk_cfg;
-    p.bulk_kc = h->bulk_kc;
-    p.rDH = (float)(1.0 / h->d.DH);
-    p.DT = DT;
-    p.Crho = (float)h->Crho;
+    StageParams p = domain_stage_params(h);
     for (int n = 0; n < nsteps; ++n) {
         int FB1 = FB_TABLE[h->it % 8][0], FB2 = FB_TABLE[h->it % 8][1], FB3 = FB_TABLE[h->it % 8][2];
         if (h->srcMode == 1)
             LCHK(launch_point_source(h->prec, h->W[0], h->W[1], h->W[2], h->lay, h->SX, h->SY, h->SZ, h->srcCoef,
-                                     ricker_moment(h->fc, h->it, DT), h->stream));
-        if (h->srcMode == 2 && h->multiActive) {
-            // addMomenteRate (addMoment.cpp:238-264) at irk = 0
-            const float tmpT = ((float)h->it + 0.0f) * DT;
-            const int srcIt = int(tmpT / h->sdt);
-            if (srcIt + 1 >= h->snt) {
-                h->multiActive = false;
-            } else {
-                const float t1 = float(srcIt) * h->sdt;
-                const float tw = (tmpT - t1) / h->sdt;
-                LCHK(launch_multi_source(h->prec, h->W[0], h->lay, h->srcIdx, h->mrate, h->npts, srcIt, tw, DT,
-                                         h->stream));
-            }
+                                     h->fc, h->it, DT, h->stream));
+        if (h->srcMode == 2)  // addMomenteRate (addMoment.cpp:238-264) at irk = 0
+            LCHK(launch_multi_source(h->prec, h->W[0], h->lay, h->srcIdx, h->mrate, h->npts, h->snt, h->sdt, h->it,
+                                     DT, h->stream));
+        if (h->bulk_cfg < 0) {
+            p.FB1 = FB1; p.FB2 = FB2; p.FB3 = FB3;
+            int rc = tune_bulk(h, p);
+            if (rc) return rc;
+            p.bulk_cfg = h->bulk_cfg;
+            p.bulk_kc = h->bulk_kc;
         }
         for (int irk = 0; irk < 4; ++irk) {
             p.Win = stage_in(h, irk);
@@ -565,29 +658,31 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
             p.irk = irk;
             p.FB1 = FB1; p.FB2 = FB2; p.FB3 = FB3;
             if (h->timing) {
-                if (h->evUsed == h->evStart.size()) {
-                    hipEvent_t a, b;
-                    HIPCHK(hipEventCreate(&a));
-                    HIPCHK(hipEventCreate(&b));
-                    h->evStart.push_back(a);
-                    h->evStop.push_back(b);
+                if (h->evUsed + 3 > h->evStart.size()) {
+                    for (int r = 0; r < 3; ++r) {
+                        hipEvent_t a;
+                        HIPCHK(hipEventCreate(&a));
+                        h->evStart.push_back(a);
+                    }
                 }
                 HIPCHK(hipEventRecord(h->evStart[h->evUsed], h->stream));
-            }
-            LCHK(launch_stage(h->prec, p, 1, h->stream));
-            if (h->timing) {
-                HIPCHK(hipEventRecord(h->evStop[h->evUsed], h->stream));
-                ++h->evUsed;
-                if (h->evUsed >= 4096) {  // fold completed events to bound their number
+                LCHK(launch_stage(h->prec, p, 1, h->stream, h->evStart[h->evUsed + 1]));
+                HIPCHK(hipEventRecord(h->evStart[h->evUsed + 2], h->stream));
+                h->evUsed += 3;
+                if (h->evUsed >= 3 * 2048) {  // fold completed events to bound their number
                     HIPCHK(hipStreamSynchronize(h->stream));
-                    for (size_t q = 0; q < h->evUsed; ++q) {
-                        float ms = 0;
-                        HIPCHK(hipEventElapsedTime(&ms, h->evStart[q], h->evStop[q]));
-                        h->tAccum += ms;
+                    for (size_t q = 0; q < h->evUsed; q += 3) {
+                        float a = 0, b = 0;
+                        HIPCHK(hipEventElapsedTime(&a, h->evStart[q], h->evStart[q + 1]));
+                        HIPCHK(hipEventElapsedTime(&b, h->evStart[q + 1], h->evStart[q + 2]));
+                        h->tBulk += a;
+                        h->tGen += b;
                         ++h->tCount;
                     }
                     h->evUsed = 0;
                 }
+            } else {
+                LCHK(launch_stage(h->prec, p, 1, h->stream, nullptr));
             }
             FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;
         }
@@ -627,7 +722,6 @@ int cgfdm_get_pgv(cgfdm_domain* h, float* out) {
     HIPCHK(hipStreamSynchronize(h->stream));
     return CGFDM_OK;
 }
-
 int cgfdm_get_slice(cgfdm_domain* h, int axis, int index, int var, float* out) {
     if (!h || !out || axis < 0 || axis > 2 || var < 0 || var > 8) { set_err("bad slice request"); return CGFDM_EINVAL; }
     const int lim = axis == 0 ? h->nx : (axis == 1 ? h->ny : h->nz);
@@ -697,20 +791,45 @@ int cgfdm_enable_timing(cgfdm_domain* h, int on) {
     return CGFDM_OK;
 }
 
-int cgfdm_stage_timing(cgfdm_domain* h, int reset, double* avg_ms, long long* count) {
+int cgfdm_kernel_timing(cgfdm_domain* h, int reset, double* out, long long* count) {
     if (!h) { set_err("null handle"); return CGFDM_EINVAL; }
     HIPCHK(hipSetDevice(h->dev));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (size_t q = 0; q < h->evUsed; ++q) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, h->evStart[q], h->evStop[q]));
-        h->tAccum += ms;
+    for (size_t q = 0; q < h->evUsed; q += 3) {
+        float a = 0, b = 0;
+        HIPCHK(hipEventElapsedTime(&a, h->evStart[q], h->evStart[q + 1]));
+        HIPCHK(hipEventElapsedTime(&b, h->evStart[q + 1], h->evStart[q + 2]));
+        h->tBulk += a;
+        h->tGen += b;
         ++h->tCount;
     }
     h->evUsed = 0;
-    if (avg_ms) *avg_ms = h->tCount ? h->tAccum / h->tCount : 0.0;
+    const double n = h->tCount ? (double)h->tCount : 1.0;
+    if (out) {
+        out[0] = h->tCount ? (h->tBulk + h->tGen) / n : 0.0;
+        out[1] = h->tCount ? h->tBulk / n : 0.0;
+        out[2] = h->tCount ? h->tGen / n : 0.0;
+    }
     if (count) *count = h->tCount;
-    if (reset) { h->tAccum = 0; h->tCount = 0; }
+    if (reset) { h->tBulk = 0; h->tGen = 0; h->tCount = 0; }
+    return CGFDM_OK;
+}
+
+int cgfdm_stage_timing(cgfdm_domain* h, int reset, double* avg_ms, long long* count) {
+    double t[3];
+    int rc = cgfdm_kernel_timing(h, reset, t, count);
+    if (rc == 0 && avg_ms) *avg_ms = t[0];
+    return rc;
+}
+
+int cgfdm_partition(const cgfdm_domain* h, long long* out) {
+    if (!h || !out) { set_err("null argument"); return CGFDM_EINVAL; }
+    StageParams q = domain_stage_params(h);
+    stage_boxes(q, 1);
+    out[0] = q.bulk_on ? (long long)(q.bi1 - q.bi0) * (q.bj1 - q.bj0) * (q.bk1 - q.bk0) : 0;
+    out[1] = q.boxStart[q.nbox];
+    out[2] = h->bulk_cfg;
+    out[3] = h->bulk_kc;
     return CGFDM_OK;
 }
 

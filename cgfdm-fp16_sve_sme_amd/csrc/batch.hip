@@ -1,0 +1,239 @@
+// batch.hip — multi-domain execution: many independent simulations advanced together, one
+// launch per kernel for all of them.
+//
+// Every kernel here is persistent over a fixed grid and reads what it works on from device
+// tables (slot table, per-slot stage parameters, tile and chunk lists). A sequence of whole time
+// steps built from these launches can therefore be captured once in a HIP graph and replayed
+// for any set of resident domains: admitting or retiring a domain only rewrites the tables
+// (k_write_slot, k_clear_slot, k_build_tables, all outside the graph) and never changes a grid.
+// Each domain keeps its own step counter on the device; k_advance moves it and the MacCormack
+// signs of the next step, and a domain whose counter reached its NT is skipped by every kernel.
+//
+// The per-point arithmetic is the single-domain code (bulk_tile, stage_point_general and the
+// side bodies), so a domain computes bit-identical results alone and in any mix.
+#include "batch.hpp"
+#include "side.hpp"
+#include "stage_bulk.hpp"
+#include "stage_general.hpp"
+
+namespace cg {
+
+namespace {
+
+__device__ const int kFB[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
+                                  {-1, 1, -1},  {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};  // propagate.cpp:107-117
+
+// Stage input/output buffers by RK stage: A -> B -> C -> B -> A (T accumulates).
+__device__ __forceinline__ int stage_in_idx(int irk) { return irk == 0 ? 0 : (irk == 2 ? 2 : 1); }
+__device__ __forceinline__ int stage_out_idx(int irk) { return irk == 0 ? 1 : (irk == 1 ? 2 : (irk == 2 ? 1 : 0)); }
+
+__device__ __forceinline__ void set_fb(StageParams& p, int it, int irk) {
+    const int sgn = (irk & 1) ? -1 : 1;
+    p.FB1 = kFB[it % 8][0] * sgn;
+    p.FB2 = kFB[it % 8][1] * sgn;
+    p.FB3 = kFB[it % 8][2] * sgn;
+}
+
+// Uniform binary search: last entry whose start <= t (entries sorted by start, n >= 1).
+template <class E, class T>
+__device__ __forceinline__ int find_entry(const E* __restrict__ e, int n, T t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (e[mid].start <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- table maintenance
+__global__ void k_write_slot(BatchDev b, int s, SlotDev sd, StageParams base) {
+    const int irk = threadIdx.x;
+    if (irk < 4) {
+        StageParams p = base;
+        p.irk = irk;
+        p.Win = sd.W[stage_in_idx(irk)];
+        p.Wout = sd.W[stage_out_idx(irk)];
+        set_fb(p, sd.it, irk);
+        b.sp[s * 4 + irk] = p;
+    }
+    if (irk == 0) b.slots[s] = sd;
+}
+
+__global__ void k_clear_slot(BatchDev b, int s) {
+    if (threadIdx.x == 0) b.slots[s].active = 0;
+}
+
+// Prefix sums of bulk tiles and general chunks over the active slots, in slot order.
+__global__ void k_build_tables(BatchDev b) {
+    if (threadIdx.x != 0) return;
+    int nb = 0, ng = 0, tiles = 0;
+    long long chunks = 0;
+    for (int s = 0; s < b.maxSlots; ++s) {
+        const SlotDev& sd = b.slots[s];
+        if (!sd.active) continue;
+        if (sd.gx > 0) {
+            BulkEnt& e = b.tab->bulk[nb++];
+            e.slot = s; e.gx = sd.gx; e.gy = sd.gy; e.nch = sd.nch; e.kc = sd.kc; e.start = tiles;
+            tiles += sd.gx * sd.gy * sd.nch;
+        }
+        for (int x = 0; x < sd.nbox; ++x) {
+            GenEnt& e = b.tab->gen[ng++];
+            e.slot = s; e.box = x; e.start = chunks;
+            chunks += sd.boxChunks[x];
+        }
+    }
+    b.tab->nBulk = nb;
+    b.tab->nTiles = tiles;
+    b.tab->nGen = ng;
+    b.tab->nChunks = chunks;
+}
+
+// ---------------------------------------------------------------- per-step kernels
+template <class S>
+__global__ void k_source_multi(BatchDev b) {
+    const SlotDev& sd = b.slots[blockIdx.y];
+    if (!sd.active || sd.it >= sd.NT) return;
+    S* A = (S*)sd.W[0];
+    if (sd.srcMode == 1) {
+        const int q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q < 343)
+            point_source_at<S>(A, (S*)sd.W[1], (S*)sd.W[2], sd.lay, sd.SX, sd.SY, sd.SZ, sd.coef,
+                               ricker_value(sd.fc, sd.it, sd.DT), q);
+    } else if (sd.srcMode == 2) {
+        int srcIt;
+        float tw;
+        if (!table_time(sd.it, sd.DT, sd.sdt, sd.snt, srcIt, tw)) return;
+        for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < sd.npts;
+             p += (long long)gridDim.x * blockDim.x)
+            table_source_at<S>(A, sd.lay, sd.srcIdx, sd.mrate, sd.npts, srcIt, tw, sd.DT, p);
+    }
+}
+
+template <class S, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b, int irk) {
+    __shared__ BulkLds<TX, TY> lds;
+    const int nEnt = b.tab->nBulk, nTiles = b.tab->nTiles;
+    for (int t = blockIdx.x; t < nTiles; t += gridDim.x) {
+        const BulkEnt& e = b.tab->bulk[find_entry(b.tab->bulk, nEnt, t)];
+        const SlotDev& sd = b.slots[e.slot];
+        if (sd.it >= sd.NT) continue;
+        const StageParams& p = b.sp[e.slot * 4 + irk];
+        const int r = t - e.start;
+        const int bx = r % e.gx, by = (r / e.gx) % e.gy, ch = r / (e.gx * e.gy);
+        const int kb = p.bk0 + ch * e.kc;
+        const int ke = min(kb + e.kc, p.bk1);
+        bulk_tile<S, TX, TY>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
+        __syncthreads();  // the next tile rewrites the LDS planes
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(kGenChunk) k_general_multi(BatchDev b, int irk) {
+    const int nEnt = b.tab->nGen;
+    const long long nChunks = b.tab->nChunks;
+    for (long long c = blockIdx.x; c < nChunks; c += gridDim.x) {
+        const GenEnt& e = b.tab->gen[find_entry(b.tab->gen, nEnt, c)];
+        const SlotDev& sd = b.slots[e.slot];
+        if (sd.it >= sd.NT) continue;
+        const StageParams& p = b.sp[e.slot * 4 + irk];
+        const int* B = p.box[e.box];
+        const long long r = (c - e.start) * kGenChunk + threadIdx.x;
+        const int nxb = B[1] - B[0], nyb = B[3] - B[2];
+        const long long n = (long long)nxb * nyb * (B[5] - B[4]);
+        if (r < n) {
+            const int i = B[0] + (int)(r % nxb);
+            const int j = B[2] + (int)((r / nxb) % nyb);
+            const int k = B[4] + (int)(r / ((long long)nxb * nyb));
+            stage_point_general<S>(p, i, j, k);
+        }
+    }
+}
+
+template <class S>
+__global__ void k_record_multi(BatchDev b) {
+    const SlotDev& sd = b.slots[blockIdx.y];
+    if (!sd.active || sd.it >= sd.NT) return;
+    const S* A = (const S*)sd.W[0];
+    const int stride = gridDim.x * blockDim.x;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int t = t0; t < sd.nst * 9; t += stride) station_at<S>(A, sd.lay, sd.stIdx, sd.stWave, sd.NT, sd.it, sd.rCv, sd.rCs, t);
+    if (sd.fs)
+        for (int t = t0; t < sd.nx * sd.ny; t += stride) pgv_at<S>(A, sd.lay, sd.nx, sd.ktop, sd.pgv, sd.rCv, t);
+}
+
+// One thread per slot: the step counter and the MacCormack signs of the next step.
+__global__ void k_advance(BatchDev b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.maxSlots) return;
+    SlotDev& sd = b.slots[s];
+    if (!sd.active || sd.it >= sd.NT) return;
+    const int it = sd.it + 1;
+    sd.it = it;
+    for (int irk = 0; irk < 4; ++irk) set_fb(b.sp[s * 4 + irk], it, irk);
+}
+
+// ---------------------------------------------------------------- launchers
+int batch_write_slot(const BatchDev& b, int s, const SlotDev& sd, const StageParams& base, hipStream_t st) {
+    k_write_slot<<<1, 64, 0, st>>>(b, s, sd, base);
+    k_build_tables<<<1, 64, 0, st>>>(b);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int batch_clear_slot(const BatchDev& b, int s, hipStream_t st) {
+    k_clear_slot<<<1, 64, 0, st>>>(b, s);
+    k_build_tables<<<1, 64, 0, st>>>(b);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <class S, int TX, int TY>
+static int batch_step_t(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+    k_source_multi<S><<<dim3(2, b.maxSlots), 256, 0, st>>>(b);
+    for (int irk = 0; irk < 4; ++irk) {
+        k_bulk_multi<S, TX, TY><<<g.bulkBlocks, TX * TY, 0, st>>>(b, irk);
+        k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, irk);
+    }
+    k_record_multi<S><<<dim3(16, b.maxSlots), 256, 0, st>>>(b);
+    k_advance<<<(b.maxSlots + 63) / 64, 64, 0, st>>>(b);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+    switch (prec * 2 + (cfg ? 1 : 0)) {
+        case P_F32 * 2: return batch_step_t<float, 64, 8>(b, g, st);
+        case P_F32 * 2 + 1: return batch_step_t<float, 64, 4>(b, g, st);
+        case P_F16 * 2: return batch_step_t<_Float16, 64, 8>(b, g, st);
+        case P_F16 * 2 + 1: return batch_step_t<_Float16, 64, 4>(b, g, st);
+        case P_BF16 * 2: return batch_step_t<__bf16, 64, 8>(b, g, st);
+        case P_BF16 * 2 + 1: return batch_step_t<__bf16, 64, 4>(b, g, st);
+    }
+    return -1;
+}
+
+// Persistent grid sizes: as many workgroups as fit on the device at once.
+int batch_grid(int prec, int cfg, BatchGrid& g) {
+    int dev = 0, ncu = 0, nb = 0, ng = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -3;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -3;
+    hipError_t e1 = hipSuccess, e2 = hipSuccess;
+    const int bt = cfg ? 64 * 4 : 64 * 8;
+#define CG_OCC(S)                                                                                                 \
+    e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4>, bt, 0)                  \
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8>, bt, 0);                 \
+    e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);
+    switch (prec) {
+        case P_F32: CG_OCC(float) break;
+        case P_F16: CG_OCC(_Float16) break;
+        case P_BF16: CG_OCC(__bf16) break;
+        default: return -1;
+    }
+#undef CG_OCC
+    if (e1 != hipSuccess || e2 != hipSuccess) return -3;
+    g.bulkBlocks = ncu * (nb > 0 ? nb : 1);
+    g.genBlocks = ncu * (ng > 0 ? ng : 1);
+    return 0;
+}
+
+}  // namespace cg

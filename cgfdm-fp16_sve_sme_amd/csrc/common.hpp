@@ -145,23 +145,38 @@ struct SetupParams {
     double Cv, Cs, Crho;
 };
 
+// Source of the medium for the setup kernels: 3-D arrays on the interior grid (vs != null), or a
+// layer table of nl rows [top depth (m) | Vs | Vp | rho] stored as four arrays of nl floats.
+struct MediumSrc {
+    const float* vs;
+    const float* vp;
+    const float* rho;
+    const float* layers;
+    int nl;
+};
+
 enum Prec { P_F32 = 1, P_F16 = 2, P_BF16 = 3 };
 
 // ---- launchers (defined in the .hip files) ----
-int launch_setup_medium(int prec, const SetupParams& q, const float* vs, const float* vp, const float* rho,
-                        void* cjm, hipStream_t s);
+int launch_setup_medium(int prec, const SetupParams& q, const MediumSrc& ms, void* cjm, hipStream_t s);
+int launch_vpmax(const SetupParams& q, const MediumSrc& ms, unsigned* out, hipStream_t s);
 int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s);
-int launch_setup_rdz(const SetupParams& q, const float* vs, const float* vp, const float* rho, float* rdz,
-                     hipStream_t s);
+int launch_setup_rdz(const SetupParams& q, const MediumSrc& ms, float* rdz, hipStream_t s);
 int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 
-int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s);
+// One RK stage: the bulk kernel (use_fast != 0) then the general kernel over the remaining
+// boxes (skipped when use_fast == 2, used to time the bulk kernel alone). `mid`, if given, is
+// recorded between the two launches.
+int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid);
+void stage_boxes(StageParams& q, int use_fast);
 
 // Point lists (sources, stations) carry padded (i, j, k) triples.
 int launch_point_source(int prec, void* A, void* B, void* C, const Layout& L, int SX, int SY, int SZ,
-                        const float* coef, float sval, hipStream_t s);
+                        const float* coef, float fc, int it, float DT, hipStream_t s);
 int launch_multi_source(int prec, void* A, const Layout& L, const int* ijk, const float* mrate, long long npts,
-                        int srcIt, float tw, float DT, hipStream_t s);
+                        int snt, float sdt, int it, float DT, hipStream_t s);
+int launch_source_coef(int prec, const void* cjm, const Layout& L, int SX, int SY, int SZ, const float* amp, float DH,
+                       double CsDT, float* coef, hipStream_t s);
 int launch_scale_mrate(int prec, const void* cjm, const Layout& L, const int* ijk, float* mrate, long long npts,
                        int nt, float DH, float Cv, hipStream_t s);
 int launch_station(int prec, const void* A, const Layout& L, const int* ijk, int nst, float* wave, int NT, int it,

@@ -1,50 +1,38 @@
 // misc.hip — per-step side kernels: sources, stations, PGV, slices and state transfer.
 // Point lists carry padded (i, j, k) triples; addresses come from the Layout.
-#include "common.hpp"
+#include "side.hpp"
 
 namespace cg {
 
 static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 #define CG_LAUNCH_OK (hipGetLastError() == hipSuccess ? 0 : -3)
 
-// 7^3 Gaussian-smoothed explosive source added to Txx, Tyy, Tzz of W before stage 0
-// (load_smooth_source, singleSource.cpp:77-127). coef[q] = -amp_q/(J*DH^3)*Cs*DT is prepared on
-// the host; sval is the Ricker moment-rate value of this step. Points in the halo are also
-// added to the two stage buffers, whose halos must equal W's (they are never recomputed).
+// Explosive point source before stage 0 of step it; the Ricker value is formed on the device.
 template <class S>
 __global__ void k_point_source(S* A, S* B, S* C, Layout L, int SX, int SY, int SZ, const float* __restrict__ coef,
-                               float sval) {
+                               float fc, int it, float DT) {
     const int q = threadIdx.x;
     if (q >= 343) return;
-    const int di = q % 7 - 3, dj = (q / 7) % 7 - 3, dk = q / 49 - 3;
-    const int i = SX + di, j = SY + dj, k = SZ + dk;
-    const float value = sval * coef[q];
-    const bool halo = i < HALO || i >= L.nx_ - HALO || j < HALO || j >= L.ny_ - HALO || k < HALO || k >= L.nz_ - HALO;
-    for (int v = 3; v <= 5; ++v) {
-        const long long p = L.at(9, v, i, j, k);
-        A[p] = SV<S>::st(SV<S>::ld(A[p]) + value);
-        if (halo) {
-            B[p] = SV<S>::st(SV<S>::ld(B[p]) + value);
-            C[p] = SV<S>::st(SV<S>::ld(C[p]) + value);
-        }
-    }
+    point_source_at<S>(A, B, C, L, SX, SY, SZ, coef, ricker_value(fc, it, DT), q);
 }
 
-// Linear-in-time interpolation of the moment-rate table and point injection into the six
-// stresses (interp_momentRate + addSource1, addMoment.cpp:98-144, 194-219).
+// Moment-rate table source before stage 0 of step it.
 template <class S>
 __global__ void k_multi_source(S* A, Layout L, const int* __restrict__ ijk, const float* __restrict__ mrate,
-                               long long npts, int srcIt, float tw, float DT) {
+                               long long npts, int snt, float sdt, int it, float DT) {
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
-    const float* m0 = mrate + ((long long)srcIt * npts + p) * 6;
-    const float* m1 = mrate + ((long long)(srcIt + 1) * npts + p) * 6;
-    for (int c = 0; c < 6; ++c) {
-        const float slice = (m1[c] - m0[c]) * tw + m0[c];
-        const long long q = L.at(9, 3 + c, i, j, k);
-        A[q] = SV<S>::st(SV<S>::ld(A[q]) - slice * DT);
-    }
+    int srcIt;
+    float tw;
+    if (p >= npts || !table_time(it, DT, sdt, snt, srcIt, tw)) return;
+    table_source_at<S>(A, L, ijk, mrate, npts, srcIt, tw, DT, p);
+}
+
+// Coefficients of the 7^3 point-source stencil from the stored Jacobian (no host round trip).
+template <class S>
+__global__ void k_source_coef(const S* __restrict__ cjm, Layout L, int SX, int SY, int SZ,
+                              const float* __restrict__ amp, float DH, double CsDT, float* __restrict__ coef) {
+    const int q = threadIdx.x;
+    if (q < 343) coef[q] = source_coef_at<S>(cjm, L, SX, SY, SZ, amp[q], DH, CsDT);
 }
 
 // One-time scaling of the table by mu*Cv/(J*DH^3) read from the stored CJM
@@ -64,30 +52,21 @@ __global__ void k_scale_mrate(const S* __restrict__ cjm, Layout L, const int* __
     for (int c = 0; c < 6; ++c) mrate[t * 6 + c] *= mu * V;
 }
 
-// Station sampling after the step (storage_station, station.cpp:309-347).
+// Station sampling after step it.
 template <class S>
 __global__ void k_station(const S* __restrict__ A, Layout L, const int* __restrict__ ijk, int nst, float* wave,
                           int NT, int it, float c1, float c2) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nst * 9) return;
-    const int s = t / 9, v = t % 9;
-    const long long pos = it + (long long)s * NT;
-    wave[pos * 9 + v] = SV<S>::ld(A[L.at(9, v, ijk[3 * s], ijk[3 * s + 1], ijk[3 * s + 2])]) * (v < 3 ? c1 : c2);
+    station_at<S>(A, L, ijk, wave, NT, it, c1, c2, t);
 }
 
-// Running maximum of horizontal and total speed on the surface layer (compare_pgv, PGV.cpp:48-101).
+// PGV update after a step.
 template <class S>
 __global__ void k_pgv(const S* __restrict__ A, Layout L, int nx, int ny, int k, float* __restrict__ pgv, float c) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx * ny) return;
-    const int i = t % nx + HALO, j = t / nx + HALO;
-    const float Vx = SV<S>::ld(A[L.at(9, 0, i, j, k)]) * c;
-    const float Vy = SV<S>::ld(A[L.at(9, 1, i, j, k)]) * c;
-    const float Vz = SV<S>::ld(A[L.at(9, 2, i, j, k)]) * c;
-    const float Vh = sqrtf(Vx * Vx + Vy * Vy);
-    const float V = sqrtf(Vx * Vx + Vy * Vy + Vz * Vz);
-    if (pgv[t * 2 + 0] < Vh) pgv[t * 2 + 0] = Vh;
-    if (pgv[t * 2 + 1] < V) pgv[t * 2 + 1] = V;
+    pgv_at<S>(A, L, nx, k, pgv, c, t);
 }
 
 // Plane gather with unscaling (pack_iodata_x/y/z, data_io.cpp:86-231).
@@ -153,20 +132,27 @@ __global__ void k_decode(const S* __restrict__ src, float* __restrict__ dst, lon
     }
 
 int launch_point_source(int prec, void* A, void* B, void* C, const Layout& L, int SX, int SY, int SZ,
-                        const float* coef, float sval, hipStream_t s) {
+                        const float* coef, float fc, int it, float DT, hipStream_t s) {
     switch (prec) {
-        case P_F32: k_point_source<float><<<1, 384, 0, s>>>((float*)A, (float*)B, (float*)C, L, SX, SY, SZ, coef, sval); break;
-        case P_F16: k_point_source<_Float16><<<1, 384, 0, s>>>((_Float16*)A, (_Float16*)B, (_Float16*)C, L, SX, SY, SZ, coef, sval); break;
-        case P_BF16: k_point_source<__bf16><<<1, 384, 0, s>>>((__bf16*)A, (__bf16*)B, (__bf16*)C, L, SX, SY, SZ, coef, sval); break;
+        case P_F32: k_point_source<float><<<1, 384, 0, s>>>((float*)A, (float*)B, (float*)C, L, SX, SY, SZ, coef, fc, it, DT); break;
+        case P_F16: k_point_source<_Float16><<<1, 384, 0, s>>>((_Float16*)A, (_Float16*)B, (_Float16*)C, L, SX, SY, SZ, coef, fc, it, DT); break;
+        case P_BF16: k_point_source<__bf16><<<1, 384, 0, s>>>((__bf16*)A, (__bf16*)B, (__bf16*)C, L, SX, SY, SZ, coef, fc, it, DT); break;
         default: return -1;
     }
     return CG_LAUNCH_OK;
 }
 
 int launch_multi_source(int prec, void* A, const Layout& L, const int* ijk, const float* mrate, long long npts,
-                        int srcIt, float tw, float DT, hipStream_t s) {
+                        int snt, float sdt, int it, float DT, hipStream_t s) {
     CG_DISPATCH(prec, k_multi_source, nblk(npts, 256), 256, s, (float*)A, (_Float16*)A, (__bf16*)A, L, ijk, mrate,
-                npts, srcIt, tw, DT);
+                npts, snt, sdt, it, DT);
+    return CG_LAUNCH_OK;
+}
+
+int launch_source_coef(int prec, const void* cjm, const Layout& L, int SX, int SY, int SZ, const float* amp, float DH,
+                       double CsDT, float* coef, hipStream_t s) {
+    CG_DISPATCH(prec, k_source_coef, 1, 384, s, (const float*)cjm, (const _Float16*)cjm, (const __bf16*)cjm, L, SX,
+                SY, SZ, amp, DH, CsDT, coef);
     return CG_LAUNCH_OK;
 }
 

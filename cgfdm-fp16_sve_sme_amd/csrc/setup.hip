@@ -72,18 +72,69 @@ __device__ __forceinline__ void medium_at(const SetupParams& q, float vs, float 
     buoy = 1.0 / r / q.Crho;
 }
 
+// Medium at interior point (i0, j0, k0): from the 3-D arrays when given, otherwise from the layer
+// table by the reference's rule for a layered model following the terrain
+// (generate_1D_medium with LayeredStructureTerrain, medium.cpp:84-216): start from the
+// homogeneous default, then for every interface F the top layer is taken above -H[0] + h, layer
+// F-1 between -H[F-1] + h and -H[F] + h, layer F below -H[F] + h (1 m tolerance, float compares),
+// the last match winning.
+__device__ __forceinline__ void medium_sample(const SetupParams& q, const MediumSrc& ms, int i0, int j0, int k0,
+                                              float& vs, float& vp, float& rho) {
+    if (ms.vs) {
+        const long long t = i0 + (long long)j0 * q.nx + (long long)k0 * q.nx * q.ny;
+        vs = ms.vs[t];
+        vp = ms.vp[t];
+        rho = ms.rho[t];
+        return;
+    }
+    double c[3];
+    coord_at(q, i0 + HALO, j0 + HALO, k0 + HALO, c);
+    const float cz = (float)c[2];
+    const double ter = q.use_terrain ? q.terrain[(i0 + HALO) + (long long)(j0 + HALO) * q.nx_] : 0.0;
+    const float* H = ms.layers;
+    const float* VS = H + ms.nl;
+    const float* VP = VS + ms.nl;
+    const float* RH = VP + ms.nl;
+    vs = 3464.0f;
+    vp = 6000.0f;
+    rho = 2670.0f;
+    const float tol = 1.0f;
+    for (int F = 1; F < ms.nl; ++F) {
+        const float h0 = (float)(-(double)H[0] + ter), h1 = (float)(-(double)H[F - 1] + ter),
+                    h2 = (float)(-(double)H[F] + ter);
+        if (cz > h0 - tol) { vs = VS[0]; vp = VP[0]; rho = RH[0]; }
+        if (h1 - tol > cz && cz > h2 - tol) { vs = VS[F - 1]; vp = VP[F - 1]; rho = RH[F - 1]; }
+        if (h2 - tol > cz) { vs = VS[F]; vp = VP[F]; rho = RH[F]; }
+    }
+}
+
 template <class S>
-__global__ void k_medium(SetupParams q, const float* __restrict__ vs, const float* __restrict__ vp,
-                         const float* __restrict__ rho, S* __restrict__ cjm) {
+__global__ void k_medium(SetupParams q, MediumSrc ms, S* __restrict__ cjm) {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     long long n = (long long)q.nx * q.ny * q.nz;
     if (t >= n) return;
     int i = (int)(t % q.nx), j = (int)((t / q.nx) % q.ny), k = (int)(t / ((long long)q.nx * q.ny));
+    float vs, vp, rho;
+    medium_sample(q, ms, i, j, k, vs, vp, rho);
     double mu, lambda, buoy;
-    medium_at(q, vs[t], vp[t], rho[t], mu, lambda, buoy);
+    medium_at(q, vs, vp, rho, mu, lambda, buoy);
     cjm[q.lay.at(13, 10, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)mu);
     cjm[q.lay.at(13, 11, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)lambda);
     cjm[q.lay.at(13, 12, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)buoy);
+}
+
+// Largest Vp of the medium (for the CFL check), as an atomic maximum on the float bits.
+__global__ void k_vpmax(SetupParams q, MediumSrc ms, unsigned* out) {
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long n = (long long)q.nx * q.ny * q.nz;
+    float vp = 0.f;
+    if (t < n) {
+        int i = (int)(t % q.nx), j = (int)((t / q.nx) % q.ny), k = (int)(t / ((long long)q.nx * q.ny));
+        float vs, rho;
+        medium_sample(q, ms, i, j, k, vs, vp, rho);
+    }
+    for (int off = 32; off > 0; off >>= 1) vp = fmaxf(vp, __shfl_down(vp, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(vp));
 }
 
 // Metric and J on the interior and, with a free surface, the mirrored values above it
@@ -106,16 +157,16 @@ __global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
 
 // Free-surface matrices -rDZ^-1 DX and -rDZ^-1 DY at the surface layer
 // (contravariant.cpp:257-351, including the reference's chi = 2*lambda + mu).
-__global__ void k_rdz(SetupParams q, const float* __restrict__ vs, const float* __restrict__ vp,
-                      const float* __restrict__ rho, float* __restrict__ rdz) {
+__global__ void k_rdz(SetupParams q, MediumSrc ms, float* __restrict__ rdz) {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)q.nx * q.ny) return;
     int i0 = (int)(t % q.nx), j0 = (int)(t / q.nx);
     int i = i0 + HALO, j = j0 + HALO, k = q.nz_ - HALO - 1;
-    long long p = i0 + (long long)j0 * q.nx + (long long)(q.nz - 1) * q.nx * q.ny;
     double m[9], J, mu, lambda, buoy;
     metric_at(q, i, j, k, m, J);
-    medium_at(q, vs[p], vp[p], rho[p], mu, lambda, buoy);
+    float vs, vp, rho;
+    medium_sample(q, ms, i0, j0, q.nz - 1, vs, vp, rho);
+    medium_at(q, vs, vp, rho, mu, lambda, buoy);
     const double xi_x = m[0], xi_y = m[1], xi_z = m[2], et_x = m[3], et_y = m[4], et_z = m[5];
     const double zt_x = m[6], zt_y = m[7], zt_z = m[8];
     const double chi = 2.0 * lambda + mu, lam_mu = lambda + mu;
@@ -201,16 +252,21 @@ __global__ void k_hmin(SetupParams q, unsigned long long* out) {
 
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int launch_setup_medium(int prec, const SetupParams& q, const float* vs, const float* vp, const float* rho, void* cjm,
-                        hipStream_t s) {
+int launch_setup_medium(int prec, const SetupParams& q, const MediumSrc& ms, void* cjm, hipStream_t s) {
     long long n = (long long)q.nx * q.ny * q.nz;
     const int bs = 256;
     switch (prec) {
-        case P_F32: k_medium<float><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (float*)cjm); break;
-        case P_F16: k_medium<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (_Float16*)cjm); break;
-        case P_BF16: k_medium<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, vs, vp, rho, (__bf16*)cjm); break;
+        case P_F32: k_medium<float><<<blocks_for(n, bs), bs, 0, s>>>(q, ms, (float*)cjm); break;
+        case P_F16: k_medium<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, ms, (_Float16*)cjm); break;
+        case P_BF16: k_medium<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, ms, (__bf16*)cjm); break;
         default: return -1;
     }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_vpmax(const SetupParams& q, const MediumSrc& ms, unsigned* out, hipStream_t s) {
+    long long n = (long long)q.nx * q.ny * q.nz;
+    k_vpmax<<<blocks_for(n, 256), 256, 0, s>>>(q, ms, out);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -227,10 +283,9 @@ int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_setup_rdz(const SetupParams& q, const float* vs, const float* vp, const float* rho, float* rdz,
-                     hipStream_t s) {
+int launch_setup_rdz(const SetupParams& q, const MediumSrc& ms, float* rdz, hipStream_t s) {
     long long n = (long long)q.nx * q.ny;
-    k_rdz<<<blocks_for(n, 256), 256, 0, s>>>(q, vs, vp, rho, rdz);
+    k_rdz<<<blocks_for(n, 256), 256, 0, s>>>(q, ms, rdz);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -33,15 +33,14 @@ static void add_box(StageParams& q, int i0, int i1, int j0, int j1, int k0, int 
     ++q.nbox;
 }
 
-template <class S>
-static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s) {
-    StageParams q = p;
-    const int lo = HALO, hiI = HALO + p.nx, hiJ = HALO + p.ny, hiK = HALO + p.nz;
-    const int np = p.pml ? p.nPML : 0;
+// Bulk box and the up to six boxes around it that the general kernel covers.
+void stage_boxes(StageParams& q, int use_fast) {
+    const int lo = HALO, hiI = HALO + q.nx, hiJ = HALO + q.ny, hiK = HALO + q.nz;
+    const int np = q.pml ? q.nPML : 0;
     q.bi0 = lo + np; q.bi1 = hiI - np;
     q.bj0 = lo + np; q.bj1 = hiJ - np;
     q.bk0 = lo + np;
-    q.bk1 = p.fs ? hiK - HALO : hiK - np;
+    q.bk1 = q.fs ? hiK - HALO : hiK - np;
     q.bulk_on = use_fast && q.bi1 > q.bi0 && q.bj1 > q.bj0 && q.bk1 > q.bk0;
     q.nbox = 0;
     q.boxStart[0] = 0;
@@ -55,6 +54,12 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s) {
     } else {
         add_box(q, lo, hiI, lo, hiJ, lo, hiK);
     }
+}
+
+template <class S>
+static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid) {
+    StageParams q = p;
+    stage_boxes(q, use_fast);
     if (q.bulk_on) {
         const int cfg = (p.bulk_cfg >= 0 && p.bulk_cfg < 2) ? p.bulk_cfg : 0;
         const int TX = kBulkCfg[cfg].tx, TY = kBulkCfg[cfg].ty;
@@ -72,19 +77,20 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s) {
         else k_stage_bulk<S, 64, 4><<<dim3(gx, gy, nch), 64 * 4, 0, s>>>(q, kc);
         if (hipGetLastError() != hipSuccess) return -3;
     }
+    if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;
     const long long n = q.boxStart[q.nbox];
-    if (n > 0) {
+    if (n > 0 && use_fast != 2) {
         const int bs = 256;
         k_stage_general<S><<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(q);
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s) {
+int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid) {
     switch (prec) {
-        case P_F32: return launch_stage_t<float>(p, use_fast, s);
-        case P_F16: return launch_stage_t<_Float16>(p, use_fast, s);
-        case P_BF16: return launch_stage_t<__bf16>(p, use_fast, s);
+        case P_F32: return launch_stage_t<float>(p, use_fast, s, mid);
+        case P_F16: return launch_stage_t<_Float16>(p, use_fast, s, mid);
+        case P_BF16: return launch_stage_t<__bf16>(p, use_fast, s, mid);
     }
     return -1;
 }

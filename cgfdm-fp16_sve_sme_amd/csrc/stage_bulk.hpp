@@ -22,17 +22,23 @@
 
 namespace cg {
 
+template <int TX, int TY> struct BulkLds {
+    static constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and var stride
+    float v[2][9 * VS];
+};
+
+// One tile: columns [bi0 + bx*TX, +TX) x [bj0 + by*TY, +TY) of the bulk box, planes [kb, ke).
+// Every thread of the workgroup calls it with the same arguments.
 template <class S, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc) {
-    constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and var stride
+__device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int bx, int by, int kb,
+                                          int ke, float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+    constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
     constexpr int NT = TX * TY;
-    __shared__ float lds[2][9 * VS];
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int i0 = p.bi0 + blockIdx.x * TX, j0 = p.bj0 + blockIdx.y * TY;
+    const int i0 = p.bi0 + bx * TX, j0 = p.bj0 + by * TY;
     const int i = i0 + tx, j = j0 + ty;
-    const int FB1 = p.FB1, FB2 = p.FB2, FB3 = p.FB3;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
     const Layout L = p.lay;
     const long long plane = L.plane;
@@ -42,8 +48,6 @@ __global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc) {
     const int vs = (int)(plane * (long long)sizeof(S));  // bytes per variable plane
 
     // z chunk, traversed in the direction FB3
-    const int kb = p.bk0 + blockIdx.z * kc;
-    const int ke = min(kb + kc, p.bk1);
     const int nk = ke - kb;
     int k = FB3 > 0 ? kb : ke - 1;
 
@@ -251,6 +255,15 @@ __global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc) {
     if (rem > 1) step(I1{}, it + 1);
     if (rem > 2) step(I2{}, it + 2);
     if (rem > 3) step(I3{}, it + 3);
+}
+
+// Single-domain launch: grid (tiles in x, tiles in y, z chunks of kc planes).
+template <class S, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc) {
+    __shared__ BulkLds<TX, TY> lds;
+    const int kb = p.bk0 + blockIdx.z * kc;
+    const int ke = min(kb + kc, p.bk1);
+    bulk_tile<S, TX, TY>(p, p.FB1, p.FB2, p.FB3, blockIdx.x, blockIdx.y, kb, ke, lds.v);
 }
 
 }  // namespace cg

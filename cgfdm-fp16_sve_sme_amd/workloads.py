@@ -158,3 +158,58 @@ def surface_stations(NX, NY, NZ, n=21, margin=None):
     xs = np.linspace(margin, NX - 1 - margin, n).astype(int)
     ys = np.linspace(margin, NY - 1 - margin, n).astype(int)
     return np.array([[x, y, NZ - 1] for y in ys for x in xs], np.int32)
+
+
+# ------------------------------------------------------------------ serving request stream
+def layered_profile(n_layers, rng, vs_range=(1500.0, 4000.0), max_depth_m=15000.0):
+    """A random 1-D layered model: n_layers tops (first at 0 m, then increasing), Vs drawn in
+    vs_range and sorted to increase with depth, Vp = 1.73 Vs, rho = 1740 (Vp/1000)^0.25."""
+    tops = np.concatenate([[0.0], np.sort(rng.uniform(500.0, max_depth_m, n_layers - 1))])
+    vs = np.sort(rng.uniform(*vs_range, n_layers))
+    vp, rho = vp_rho_from_vs(vs)
+    return (tops.astype(np.float32), vs.astype(np.float32), vp.astype(np.float32),
+            rho.astype(np.float32))
+
+
+def layer_mu_at(depth_m, layers):
+    """mu = rho Vs^2 of the layer containing depth_m (below the surface)."""
+    tops, vs, vp, rho = layers
+    q = int(np.searchsorted(tops, depth_m, side="right")) - 1
+    q = max(0, min(q, len(tops) - 1))
+    return float(rho[q]) * float(vs[q]) ** 2
+
+
+def serving_stream(n, seed, rate_hz=20.0, edge_xy=(101, 301), edge_z=(51, 151), layers=(2, 5),
+                   vs_range=(1500.0, 4000.0), depth_km=(1.0, 15.0), fc=(0.5, 3.0), steps=(500, 1500),
+                   DH=100.0, DT=0.008, nPML=12, precision=2, n_st=5):
+    """Seeded Green's-function-style request stream: Poisson arrivals at rate_hz, grid edges
+    uniform in edge_xy (x, y) and edge_z (z), random layered media, random double couples at a
+    random depth (kept inside the grid above the bottom PML), Ricker moment rate of random peak
+    frequency, random step counts, an n_st x n_st surface station grid. Returns a list of
+    (arrival_s, request) with request dicts as cgfdm.Server.submit takes them (desc is built
+    by the caller from request['grid'] etc. to avoid importing the binding here)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for r in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        NX = int(rng.integers(edge_xy[0], edge_xy[1] + 1))
+        NY = int(rng.integers(edge_xy[0], edge_xy[1] + 1))
+        NZ = int(rng.integers(edge_z[0], edge_z[1] + 1))
+        nl = int(rng.integers(layers[0], layers[1] + 1))
+        lay = layered_profile(nl, rng, vs_range, max_depth_m=NZ * DH)
+        NT = int(rng.integers(steps[0], steps[1] + 1))
+        f = float(rng.uniform(*fc))
+        strike, dip, rake = float(rng.uniform(0, 360)), float(rng.uniform(10, 90)), float(rng.uniform(-180, 180))
+        dep = float(rng.uniform(*depth_km)) * 1e3
+        kmin = nPML + 4  # stay above the bottom PML with the 7-point stencil
+        Z = int(min(max(NZ - 1 - round(dep / DH), kmin), NZ - 4))
+        X = int(rng.integers(NX // 4, 3 * NX // 4))
+        Y = int(rng.integers(NY // 4, 3 * NY // 4))
+        mu = layer_mu_at((NZ - 1 - Z) * DH, lay)
+        table = double_couple_table(1e17, strike, dip, rake, f, NT + 2, DT, mu)
+        st = surface_stations(NX, NY, NZ, n=n_st)
+        out.append((t, {"grid": (NX, NY, NZ), "DH": DH, "DT": DT, "NT": NT, "nPML": nPML,
+                        "precision": precision, "layers": lay,
+                        "table": (np.array([[X, Y, Z]], np.int32), table, DT), "stations": st}))
+    return out

@@ -146,8 +146,106 @@ int cgfdm_get_cjm(cgfdm_domain* h, float* out);
  * the domain's stream since the last reset, and the number of launches timed. */
 int cgfdm_stage_timing(cgfdm_domain* h, int reset, double* avg_ms, long long* count);
 
+/* Per-kernel split of the same timing: out[0] whole stage, out[1] bulk kernel, out[2] boundary
+ * (PML and free-surface) kernel, average ms per stage; count = stages timed. */
+int cgfdm_kernel_timing(cgfdm_domain* h, int reset, double* out, long long* count);
+
 /* Enables/disables HIP-event timing of the stage kernels (off by default). */
 int cgfdm_enable_timing(cgfdm_domain* h, int on);
+
+/* How a stage is split: out[0] points of the bulk box, out[1] points of the boundary boxes,
+ * out[2] bulk tile shape (0: 64x8, 1: 64x4, -1: not tuned yet), out[3] z-chunk (0: automatic). */
+int cgfdm_partition(const cgfdm_domain* h, long long* out);
+
+/* Layered 1-D medium following the terrain, built on the device: the rule of the reference's
+ * layer-file model (medium.cpp:84-216, LayeredStructureTerrain) for nlayer rows of top depth
+ * (m, increasing), Vs, Vp (m/s) and rho (kg/m^3); points above the first interface and
+ * below none keep the homogeneous default (3464, 6000, 2670). Otherwise as cgfdm_set_model. */
+int cgfdm_set_model_layered(cgfdm_domain* h, const double* terrain, int nlayer, const float* depth,
+                            const float* vs, const float* vp, const float* rho);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving: many independent simulations on one GPU with continuous batching.
+ * No counterpart in the reference, which runs one simulation per MPI job; this is the engine
+ * for Green's-function-style request streams. A server owns one stream, one device arena of
+ * hbm_budget bytes and a pinned host arena. Requests are admitted in arrival order whenever a
+ * slot is free and the arena holds the request's whole device footprint; resident domains are
+ * advanced together, steps_per_launch whole steps per launch of one persistent multi-domain
+ * kernel set (replayed from a HIP graph when use_graph), and retired when they reach NT.
+ * Every domain computes the same numbers alone and in any mix (the per-point arithmetic does
+ * not depend on the batch).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct cgfdm_server cgfdm_server;
+
+typedef struct cgfdm_server_config {
+    int device;
+    int precision;          /* storage precision of every request                       */
+    long long hbm_budget;   /* device arena bytes; 0: 80 % of the free HBM              */
+    long long host_budget;  /* pinned host arena bytes; 0: 1 GiB                          */
+    int max_domains;        /* resident simulations, 1..64                                */
+    int steps_per_launch;   /* whole time steps per batch launch, 1..64                   */
+    int use_graph;          /* 1: replay a captured HIP graph; 0: launch directly         */
+} cgfdm_server_config;
+
+typedef struct cgfdm_request {
+    cgfdm_desc desc;            /* grid, DH, DT, NT, nPML, ...; precision = the server's;
+                                   check_cfl must be 0                                      */
+    const double* terrain;      /* (NY+6) x (NX+6) heights when desc.use_terrain            */
+    int nlayer;                 /* > 0: layered medium (as cgfdm_set_model_layered)         */
+    const float* layer_depth;
+    const float* layer_vs;
+    const float* layer_vp;
+    const float* layer_rho;
+    const float* vs;            /* nlayer == 0: NZ x NY x NX medium (as cgfdm_set_model)    */
+    const float* vp;
+    const float* rho;
+    int source;                 /* 1: explosive point (src_xyz, fc); 2: moment-rate table   */
+    int src_xyz[3];
+    float fc;
+    const int* tab_xyz;         /* as cgfdm_set_source_table                                */
+    const float* tab_rate;
+    long long tab_npts;
+    int tab_nt;
+    float tab_dt;
+    const int* st_xyz;          /* stations (nst x 3 global indices)                        */
+    int nst;
+} cgfdm_request;
+
+typedef struct cgfdm_request_info {
+    int status;                 /* CGFDM_OK or the failure of this request                 */
+    int NT, nst, NX, NY;        /* result shapes: stations nst x NT x 9, PGV NY x NX x 2     */
+    double t_submit, t_admit, t_done;  /* seconds since the server started (admit -1: never) */
+    long long device_bytes;     /* arena bytes the request held                              */
+    char error[160];
+} cgfdm_request_info;
+
+typedef struct cgfdm_server_stats {
+    long long submitted, completed, failed, launches;
+    int resident, pending, peak_resident, graph, tile_cfg, bulk_blocks, general_blocks;
+    long long arena_bytes, arena_used, arena_peak;
+} cgfdm_server_stats;
+
+/* Creates a server and starts its worker thread. */
+int cgfdm_server_create(const cgfdm_server_config* cfg, cgfdm_server** out);
+
+/* Copies the request (all arrays) and queues it; *id identifies it. Fails at once with
+ * CGFDM_ENOMEM if the request could never fit the arenas. */
+int cgfdm_server_submit(cgfdm_server* s, const cgfdm_request* r, long long* id);
+
+/* Waits up to timeout_ms (< 0: forever) for completed, not yet fetched requests and writes up
+ * to max of their ids; returns how many (0 on timeout). */
+int cgfdm_server_wait(cgfdm_server* s, long long* ids, int max, int timeout_ms);
+
+/* Copies a completed request's seismograms (nst x NT x 9, as cgfdm_get_stations) and PGV
+ * (NY x NX x 2) out (either may be NULL), fills info, releases the record and returns the
+ * request's status. */
+int cgfdm_server_result(cgfdm_server* s, long long id, float* stations, float* pgv, cgfdm_request_info* info);
+
+/* Counters of the server. */
+int cgfdm_server_stats_get(cgfdm_server* s, cgfdm_server_stats* st);
+
+/* Finishes every submitted request, stops the worker and frees everything. */
+int cgfdm_server_destroy(cgfdm_server* s);
 
 #ifdef __cplusplus
 }

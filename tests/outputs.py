@@ -82,15 +82,18 @@ def write_layer_file(path, dep_m, vs, vp, rho):
             f.write(struct.pack("ffff", d / 1000, s / 1000, p / 1000, r / 1000))
 
 
-def layered_medium(coordz, terrain_surface, dep_m, vs, vp, rho, base=(3464.0, 6000.0, 2670.0)):
+def layered_medium(coordz, terrain_surface, dep_m, vs, vp, rho, base=(3464.0, 6000.0, 2670.0), from_km=True):
     """Restatement of generate_1D_medium (medium.cpp:84-216) with LayeredStructureTerrain:
     the last matching rule among the layer tests wins; values start from the homogeneous default.
-    All comparisons in float32, as the reference's floats."""
+    All comparisons in float32, as the reference's floats. from_km: the values pass through the
+    layer file's km units (float km * 1000, as the reference reads them); otherwise they are
+    taken as float32 metres and m/s."""
     f32 = np.float32
-    H = [f32(f32(d / 1000) * f32(1000.0)) for d in dep_m]
-    VS = [f32(f32(v / 1000) * f32(1000.0)) for v in vs]
-    VP = [f32(f32(v / 1000) * f32(1000.0)) for v in vp]
-    RH = [f32(f32(v / 1000) * f32(1000.0)) for v in rho]
+    conv = (lambda v: f32(f32(v / 1000) * f32(1000.0))) if from_km else f32
+    H = [conv(d) for d in dep_m]
+    VS = [conv(v) for v in vs]
+    VP = [conv(v) for v in vp]
+    RH = [conv(v) for v in rho]
     cz = coordz.astype(np.float32)
     ter = np.broadcast_to(terrain_surface, cz.shape).astype(np.float64)
     out = [np.full(cz.shape, b, np.float32) for b in base]
