@@ -112,11 +112,15 @@ __global__ void k_source_multi(BatchDev b) {
     }
 }
 
-template <class S, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b, int irk) {
+template <class S, int TX, int TY, int IRK>
+__global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b) {
+    constexpr int irk = IRK;
     __shared__ BulkLds<TX, TY> lds;
     const int nEnt = b.tab->nBulk, nTiles = b.tab->nTiles;
-    for (int t = blockIdx.x; t < nTiles; t += gridDim.x) {
+    // the persistent grid is a multiple of the XCD count, so work index w keeps its hardware
+    // block's XCD (w % kXcds) and xcd_order gives every XCD a contiguous run of tiles
+    for (int w = blockIdx.x; w < nTiles; w += gridDim.x) {
+        const int t = (int)xcd_order(w, nTiles);
         const BulkEnt& e = b.tab->bulk[find_entry(b.tab->bulk, nEnt, t)];
         const SlotDev& sd = b.slots[e.slot];
         if (sd.it >= sd.NT) continue;
@@ -125,7 +129,7 @@ __global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b, int irk) {
         const int bx = r % e.gx, by = (r / e.gx) % e.gy, ch = r / (e.gx * e.gy);
         const int kb = p.bk0 + ch * e.kc;
         const int ke = min(kb + e.kc, p.bk1);
-        bulk_tile<S, TX, TY>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
+        bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
         __syncthreads();  // the next tile rewrites the LDS planes
     }
 }
@@ -134,7 +138,8 @@ template <class S>
 __global__ void __launch_bounds__(kGenChunk) k_general_multi(BatchDev b, int irk) {
     const int nEnt = b.tab->nGen;
     const long long nChunks = b.tab->nChunks;
-    for (long long c = blockIdx.x; c < nChunks; c += gridDim.x) {
+    for (long long w = blockIdx.x; w < nChunks; w += gridDim.x) {
+        const long long c = xcd_order(w, nChunks);
         const GenEnt& e = b.tab->gen[find_entry(b.tab->gen, nEnt, c)];
         const SlotDev& sd = b.slots[e.slot];
         if (sd.it >= sd.NT) continue;
@@ -191,10 +196,14 @@ int batch_clear_slot(const BatchDev& b, int s, hipStream_t st) {
 template <class S, int TX, int TY>
 static int batch_step_t(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
     k_source_multi<S><<<dim3(2, b.maxSlots), 256, 0, st>>>(b);
-    for (int irk = 0; irk < 4; ++irk) {
-        k_bulk_multi<S, TX, TY><<<g.bulkBlocks, TX * TY, 0, st>>>(b, irk);
-        k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, irk);
-    }
+    k_bulk_multi<S, TX, TY, 0><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 0);
+    k_bulk_multi<S, TX, TY, 1><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 1);
+    k_bulk_multi<S, TX, TY, 2><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 2);
+    k_bulk_multi<S, TX, TY, 3><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 3);
     k_record_multi<S><<<dim3(16, b.maxSlots), 256, 0, st>>>(b);
     k_advance<<<(b.maxSlots + 63) / 64, 64, 0, st>>>(b);
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -220,8 +229,8 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     hipError_t e1 = hipSuccess, e2 = hipSuccess;
     const int bt = cfg ? 64 * 4 : 64 * 8;
 #define CG_OCC(S)                                                                                                 \
-    e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4>, bt, 0)                  \
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8>, bt, 0);                 \
+    e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4, 1>, bt, 0)               \
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8, 1>, bt, 0);              \
     e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);
     switch (prec) {
         case P_F32: CG_OCC(float) break;
@@ -231,8 +240,9 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     }
 #undef CG_OCC
     if (e1 != hipSuccess || e2 != hipSuccess) return -3;
-    g.bulkBlocks = ncu * (nb > 0 ? nb : 1);
-    g.genBlocks = ncu * (ng > 0 ? ng : 1);
+    // multiples of the XCD count (see k_bulk_multi)
+    g.bulkBlocks = (ncu * (nb > 0 ? nb : 1) + kXcds - 1) / kXcds * kXcds;
+    g.genBlocks = (ncu * (ng > 0 ? ng : 1) + kXcds - 1) / kXcds * kXcds;
     return 0;
 }
 

@@ -195,4 +195,16 @@ int launch_decode(int prec, const void* src, float* dst, long long n, hipStream_
 
 inline int storage_bytes(int prec) { return prec == P_F32 ? 4 : 2; }
 
+// Workgroups of a launch are dealt to the XCDs round-robin (hardware id b runs on XCD b % 8 on
+// MI355X, each XCD with its own L2). xcd_order maps hardware id b to a work index so that each
+// XCD receives one contiguous run of work items: neighbouring tiles, whose halos overlap, then
+// share an L2 instead of fetching the same lines from HBM on different XCDs. A bijection on
+// [0, n) for any n (correctness does not depend on the XCD count, only locality does).
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ long long xcd_order(long long b, long long n) {
+    const long long q = n / kXcds, r = n % kXcds;
+    const long long x = b % kXcds, local = b / kXcds;
+    return x * q + (x < r ? x : r) + local;
+}
+
 }  // namespace cg

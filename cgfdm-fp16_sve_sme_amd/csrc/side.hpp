@@ -10,6 +10,7 @@ namespace cg {
 // Ricker moment-rate value at t = it*DT, M0 = 1e16 (sourceFunction at irk = 0,
 // singleSource.cpp:29-56), with the reference's float/double mix.
 __device__ __forceinline__ float ricker_value(float fc, int it, float DT) {
+#pragma clang fp contract(off)
     const float t = ((float)it + 0.0f) * DT;
     const float tdelay = 1.2f / fc;
     const float f0 = 2;
@@ -25,6 +26,7 @@ __device__ __forceinline__ float ricker_value(float fc, int it, float DT) {
 template <class S>
 __device__ __forceinline__ void point_source_at(S* A, S* B, S* C, const Layout& L, int SX, int SY, int SZ,
                                                 const float* __restrict__ coef, float sval, int q) {
+#pragma clang fp contract(off)
     const int di = q % 7 - 3, dj = (q / 7) % 7 - 3, dk = q / 49 - 3;
     const int i = SX + di, j = SY + dj, k = SZ + dk;
     const float value = sval * coef[q];
@@ -42,6 +44,7 @@ __device__ __forceinline__ void point_source_at(S* A, S* B, S* C, const Layout& 
 // Table time index and weight for step it (addMomenteRate, addMoment.cpp:238-264); false once
 // srcIt + 1 reaches the table end, after which the reference stops injecting.
 __device__ __forceinline__ bool table_time(int it, float DT, float sdt, int snt, int& srcIt, float& tw) {
+#pragma clang fp contract(off)
     const float tmpT = ((float)it + 0.0f) * DT;
     srcIt = int(tmpT / sdt);
     if (srcIt + 1 >= snt) return false;
@@ -56,6 +59,7 @@ template <class S>
 __device__ __forceinline__ void table_source_at(S* A, const Layout& L, const int* __restrict__ ijk,
                                                 const float* __restrict__ mrate, long long npts, int srcIt, float tw,
                                                 float DT, long long p) {
+#pragma clang fp contract(off)
     const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
     const float* m0 = mrate + ((long long)srcIt * npts + p) * 6;
     const float* m1 = mrate + ((long long)(srcIt + 1) * npts + p) * 6;
@@ -70,6 +74,7 @@ __device__ __forceinline__ void table_source_at(S* A, const Layout& L, const int
 template <class S>
 __device__ __forceinline__ void station_at(const S* __restrict__ A, const Layout& L, const int* __restrict__ ijk,
                                            float* wave, int NT, int it, float c1, float c2, int t) {
+#pragma clang fp contract(off)
     const int s = t / 9, v = t % 9;
     const long long pos = it + (long long)s * NT;
     wave[pos * 9 + v] = SV<S>::ld(A[L.at(9, v, ijk[3 * s], ijk[3 * s + 1], ijk[3 * s + 2])]) * (v < 3 ? c1 : c2);
@@ -79,6 +84,7 @@ __device__ __forceinline__ void station_at(const S* __restrict__ A, const Layout
 template <class S>
 __device__ __forceinline__ void pgv_at(const S* __restrict__ A, const Layout& L, int nx, int k, float* __restrict__ pgv,
                                        float c, int t) {
+#pragma clang fp contract(off)
     const int i = t % nx + HALO, j = t / nx + HALO;
     const float Vx = SV<S>::ld(A[L.at(9, 0, i, j, k)]) * c;
     const float Vy = SV<S>::ld(A[L.at(9, 1, i, j, k)]) * c;
@@ -95,6 +101,7 @@ __device__ __forceinline__ void pgv_at(const S* __restrict__ A, const Layout& L,
 template <class S>
 __device__ __forceinline__ float source_coef_at(const S* __restrict__ cjm, const Layout& L, int SX, int SY, int SZ,
                                                 float amp, float DH, double CsDT) {
+#pragma clang fp contract(off)
     const float jacb = SV<S>::ld(cjm[L.at(13, 9, SX, SY, SZ)]);
     const float value = -1.0f * amp / (jacb * (DH * DH * DH));
     return (float)(value * CsDT);

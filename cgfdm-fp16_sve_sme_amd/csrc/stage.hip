@@ -8,11 +8,22 @@ namespace cg {
 // Bulk tile shapes (x by y columns per workgroup); the choice is a launch parameter.
 struct BulkCfg { int tx, ty; };
 static const BulkCfg kBulkCfg[] = {{64, 8}, {64, 4}};
+constexpr int kNumBulkCfg = 2;
+
+template <class S, int TX, int TY>
+static void launch_bulk(const StageParams& q, unsigned grid, int kc, int gx, int gy, hipStream_t s) {
+    switch (q.irk) {
+        case 0: k_stage_bulk<S, TX, TY, 0><<<grid, TX * TY, 0, s>>>(q, kc, gx, gy); break;
+        case 1: k_stage_bulk<S, TX, TY, 1><<<grid, TX * TY, 0, s>>>(q, kc, gx, gy); break;
+        case 2: k_stage_bulk<S, TX, TY, 2><<<grid, TX * TY, 0, s>>>(q, kc, gx, gy); break;
+        default: k_stage_bulk<S, TX, TY, 3><<<grid, TX * TY, 0, s>>>(q, kc, gx, gy); break;
+    }
+}
 
 // General kernel over a list of boxes (flattened with prefix sums).
 template <class S>
 __global__ void __launch_bounds__(256) k_stage_general(StageParams p) {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long t = xcd_order(blockIdx.x, (long long)gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= p.boxStart[p.nbox]) return;
     int b = 0;
     while (b + 1 < p.nbox && t >= p.boxStart[b + 1]) ++b;
@@ -61,7 +72,7 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
     StageParams q = p;
     stage_boxes(q, use_fast);
     if (q.bulk_on) {
-        const int cfg = (p.bulk_cfg >= 0 && p.bulk_cfg < 2) ? p.bulk_cfg : 0;
+        const int cfg = (p.bulk_cfg >= 0 && p.bulk_cfg < kNumBulkCfg) ? p.bulk_cfg : 0;
         const int TX = kBulkCfg[cfg].tx, TY = kBulkCfg[cfg].ty;
         const int nxb = q.bi1 - q.bi0, nyb = q.bj1 - q.bj0, nzb = q.bk1 - q.bk0;
         const int gx = (nxb + TX - 1) / TX, gy = (nyb + TY - 1) / TY;
@@ -73,8 +84,9 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
         }
         kc = std::max(1, std::min(kc, nzb));
         const int nch = (nzb + kc - 1) / kc;
-        if (cfg == 0) k_stage_bulk<S, 64, 8><<<dim3(gx, gy, nch), 64 * 8, 0, s>>>(q, kc);
-        else k_stage_bulk<S, 64, 4><<<dim3(gx, gy, nch), 64 * 4, 0, s>>>(q, kc);
+        const unsigned grid = (unsigned)((long long)gx * gy * nch);
+        if (cfg == 0) launch_bulk<S, 64, 8>(q, grid, kc, gx, gy, s);
+        else launch_bulk<S, 64, 4>(q, grid, kc, gx, gy, s);
         if (hipGetLastError() != hipSuccess) return -3;
     }
     if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;

@@ -8,11 +8,16 @@
 //  - the current plane of the 9 fields, with its (4-wide) x/y halo, is staged in LDS as FP32 and
 //    double-buffered, so one barrier per plane suffices; x and y neighbours are LDS reads;
 //  - every global load of an iteration (the plane k+4FB3 entering the queue, the halo cells, the
-//    12 metric/medium values and the RK registers) is issued at the top of the iteration, before
-//    any wait, so each wave keeps ~40 loads in flight;
+//    12 metric/medium values and the RK registers) is unconditional and issued at the top of the
+//    iteration, before any wait: no load sits in a branch, where the compiler would have to wait
+//    for it inside the branch and serialise the iteration's memory latency;
 //  - global accesses are buffer loads/stores: one descriptor per array and plane (k-major
 //    [k][var][j][i] layout), a per-lane byte offset that does not change along the march and a
-//    wave-uniform per-variable offset, so no VALU address arithmetic is spent.
+//    wave-uniform per-variable offset, so no VALU address arithmetic is spent;
+//  - the RK stage is a template parameter, so the register update has no run-time branches;
+//  - the z loop is unrolled by 5 (the queue rotates by renaming) and always runs a multiple of 5
+//    iterations: the ones past the chunk load clamped planes and store nothing, so there is a
+//    single copy of the unrolled body.
 // Arithmetic is FP32 with the stencil coefficients pre-multiplied by FB*rDH and DT folded into
 // lambda, mu and the buoyancy. In the bulk every PML factor is 1, so it is not applied.
 #pragma once
@@ -29,7 +34,7 @@ template <int TX, int TY> struct BulkLds {
 
 // One tile: columns [bi0 + bx*TX, +TX) x [bj0 + by*TY, +TY) of the bulk box, planes [kb, ke).
 // Every thread of the workgroup calls it with the same arguments.
-template <class S, int TX, int TY>
+template <class S, int TX, int TY, int IRK>
 __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int bx, int by, int kb,
                                           int ke, float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
     constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
@@ -49,13 +54,17 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 
     // z chunk, traversed in the direction FB3
     const int nk = ke - kb;
-    int k = FB3 > 0 ? kb : ke - 1;
+    const int k0 = FB3 > 0 ? kb : ke - 1;
+    const int kmax = L.nz_ - 1;
 
-    const S* __restrict__ W = (const S*)p.Win;
-    const S* __restrict__ Cj = (const S*)p.cjm;
-    const S* __restrict__ Ap = (const S*)p.A;
-    S* __restrict__ Tp = (S*)p.T;
-    S* __restrict__ Wo = (S*)p.Wout;
+    const char* __restrict__ Wb = (const char*)p.Win;
+    const char* __restrict__ Cb = (const char*)p.cjm;
+    const char* __restrict__ Ab = (const char*)p.A;
+    char* __restrict__ Tb = (char*)p.T;
+    char* __restrict__ Ob = (char*)p.Wout;
+    const long long wpl = 9 * plane * (long long)sizeof(S);   // bytes per plane of W, A, T, Wout
+    const long long cpl = 13 * plane * (long long)sizeof(S);  // bytes per plane of the metric/medium
+    const long long wrange = 9 * plane, crange = 13 * plane;  // elements per plane
 
     // stencil weights with direction and 1/DH folded in
     const float rDH = p.rDH;
@@ -70,27 +79,24 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     float q[5][9];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const Rsrc rm = make_rsrc(W + (long long)(k + FB3 * (m - 1)) * 9 * plane, 9 * plane);
+        const Rsrc rm = make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange);
 #pragma unroll
         for (int v = 0; v < 9; ++v) q[m][v] = bld<S>(rm, col, v * vs);
     }
 
     // halo cells of this thread: cell c = tid + r*NT (r < HPT) of the list x-halo (4 columns x TY
-    // rows) then y-halo (4 rows x LX columns)
+    // rows) then y-halo (4 rows x LX columns); a thread without a cell loads element 0 in vain
     constexpr int NHX = 4 * TY, NHY = 4 * LX, NH = NHX + NHY;
     constexpr int HPT = (NH + NT - 1) / NT;
     int hl[HPT];
     unsigned hcol[HPT];
     bool hown[HPT];
-    bool haloWave[HPT];
 #pragma unroll
     for (int r = 0; r < HPT; ++r) {
         const int c = tid + r * NT;
         hl[r] = 0;
         hcol[r] = 0;
         hown[r] = c < NH;
-        // waves holding at least one halo cell in this round (wave-uniform)
-        haloWave[r] = __builtin_amdgcn_readfirstlane(c) < NH;
         if (c < NH) {
             int hr, hc;
             if (c < NHX) {
@@ -112,12 +118,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const int lc = (ty + hy) * LX + tx + hx;
     const float DT = p.DT;
     const float Crho = p.Crho;
-    const int irk = p.irk;
-    const int kmax = L.nz_ - 1;
+    int k = k0;
 
-    // One plane. Every load is unconditional and in bounds (lanes outside the grid read element 0
-    // of the plane and their values are never used), so the compiler issues them back to back and
-    // waits once; R rotates the queue by renaming instead of moving 45 registers.
     auto step = [&](auto Rc, int it) {
         constexpr int R = decltype(Rc)::value;
         float(&q0)[9] = q[(0 + R) % 5];
@@ -125,41 +127,40 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         float(&q2)[9] = q[(2 + R) % 5];
         float(&q3)[9] = q[(3 + R) % 5];
         float(&q4)[9] = q[(4 + R) % 5];
+        const bool active = it < nk;  // iterations past the chunk store nothing
         float* Lb = lds[it & 1];
-        const long long pw = (long long)k * 9 * plane;   // element offset of plane k of W, A, T, Wout
-        const long long pc = (long long)k * 13 * plane;  // and of the metric/medium planes
-        const Rsrc rW = make_rsrc(W + pw, 9 * plane);
+        const int kc = min(max(k, 0), kmax);  // clamped plane (only the inactive iterations need it)
+        const long long pw = (long long)kc * wpl;
+        const Rsrc rW = make_rsrc((const S*)(Wb + pw), wrange);
         float hv[HPT][9];
 #pragma unroll
         for (int r = 0; r < HPT; ++r)
-            if (haloWave[r]) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) hv[r][v] = bld<S>(rW, hcol[r], v * vs);
-            }
-        // the plane entering the queue; clamped on the last iteration (value unused there)
-        const int kn = min(max(k + 4 * FB3, 0), kmax);
+            for (int v = 0; v < 9; ++v) hv[r][v] = bld<S>(rW, hcol[r], v * vs);
+        // the plane entering the queue (clamped at the grid end; unused there)
         float nq[9];
         {
-            const Rsrc rN = make_rsrc(W + (long long)kn * 9 * plane, 9 * plane);
+            const int kn = min(max(k + 4 * FB3, 0), kmax);
+            const Rsrc rN = make_rsrc((const S*)(Wb + (long long)kn * wpl), wrange);
 #pragma unroll
             for (int v = 0; v < 9; ++v) nq[v] = bld<S>(rN, col, v * vs);
         }
         float M[12];
         {
-            const Rsrc rC = make_rsrc(Cj + pc, 13 * plane);
+            const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kc * cpl), crange);
 #pragma unroll
             for (int c = 0; c < 9; ++c) M[c] = bld<S>(rC, col, c * vs);
 #pragma unroll
             for (int c = 10; c < 13; ++c) M[c - 1] = bld<S>(rC, col, c * vs);
         }
-        const Rsrc rT = make_rsrc((const S*)Tp + pw, 9 * plane);
+        const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         float ra[9], rt[9];
-        if (irk == 1 || irk == 2) {
-            const Rsrc rA = make_rsrc(Ap + pw, 9 * plane);
+        if constexpr (IRK == 1 || IRK == 2) {
+            const Rsrc rA = make_rsrc((const S*)(Ab + pw), wrange);
 #pragma unroll
             for (int v = 0; v < 9; ++v) ra[v] = bld<S>(rA, col, v * vs);
         }
-        if (irk >= 1) {
+        if constexpr (IRK >= 1) {
 #pragma unroll
             for (int v = 0; v < 9; ++v) rt[v] = bld<S>(rT, col, v * vs);
         }
@@ -168,12 +169,12 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         for (int v = 0; v < 9; ++v) Lb[v * VS + lc] = q1[v];
 #pragma unroll
         for (int r = 0; r < HPT; ++r)
-            if (haloWave[r] && hown[r]) {
+            if (hown[r]) {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) Lb[v * VS + hl[r]] = hv[r][v];
             }
         __syncthreads();
-        if (inside) {
+        if (inside && active) {
             const float mu = M[9] * DT, lam = M[10] * DT, bu = M[11] * (Crho * DT);
             const int ox0 = lc - FB1, ox2 = lc + FB1, ox3 = lc + 2 * FB1, ox4 = lc + 3 * FB1;
             const int oy0 = lc - FB2 * LX, oy2 = lc + FB2 * LX, oy3 = lc + 2 * FB2 * LX, oy4 = lc + 3 * FB2 * LX;
@@ -214,17 +215,17 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             h[7] = mu * (G[0][2] + G[2][0]);
             h[8] = mu * (G[1][2] + G[2][1]);
             // RK registers (wave_rk0..3)
-            const Rsrc rO = make_rsrc(Wo + pw, 9 * plane);
+            const Rsrc rO = make_rsrc((const S*)(Ob + pw), wrange);
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
-                if (irk == 0) {
+                if constexpr (IRK == 0) {
                     const float m0 = q1[v];
                     bst<S>(rO, col, v * vs, m0 + alpha2 * h[v]);
                     bst<S>(rT, col, v * vs, m0 + beta1 * h[v]);
-                } else if (irk == 1) {
+                } else if constexpr (IRK == 1) {
                     bst<S>(rO, col, v * vs, ra[v] + alpha3 * h[v]);
                     bst<S>(rT, col, v * vs, rt[v] + beta2 * h[v]);
-                } else if (irk == 2) {
+                } else if constexpr (IRK == 2) {
                     bst<S>(rO, col, v * vs, ra[v] + h[v]);
                     bst<S>(rT, col, v * vs, rt[v] + beta3 * h[v]);
                 } else {
@@ -242,28 +243,26 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     using I2 = std::integral_constant<int, 2>;
     using I3 = std::integral_constant<int, 3>;
     using I4 = std::integral_constant<int, 4>;
-    int it = 0;
-    for (; it + 5 <= nk; it += 5) {
+    for (int it = 0; it < nk; it += 5) {
         step(I0{}, it);
         step(I1{}, it + 1);
         step(I2{}, it + 2);
         step(I3{}, it + 3);
         step(I4{}, it + 4);
     }
-    const int rem = nk - it;
-    if (rem > 0) step(I0{}, it);
-    if (rem > 1) step(I1{}, it + 1);
-    if (rem > 2) step(I2{}, it + 2);
-    if (rem > 3) step(I3{}, it + 3);
 }
 
-// Single-domain launch: grid (tiles in x, tiles in y, z chunks of kc planes).
-template <class S, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc) {
+// Single-domain launch: a 1-D grid of gx*gy*nch workgroups (x tiles fastest, then y tiles, then
+// z chunks of kc planes), dealt to the XCDs in contiguous runs (xcd_order). One instantiation per
+// RK stage.
+template <class S, int TX, int TY, int IRK>
+__global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc, int gx, int gy) {
     __shared__ BulkLds<TX, TY> lds;
-    const int kb = p.bk0 + blockIdx.z * kc;
+    const long long t = xcd_order(blockIdx.x, (long long)gridDim.x);
+    const int bx = (int)(t % gx), by = (int)((t / gx) % gy), bz = (int)(t / ((long long)gx * gy));
+    const int kb = p.bk0 + bz * kc;
     const int ke = min(kb + kc, p.bk1);
-    bulk_tile<S, TX, TY>(p, p.FB1, p.FB2, p.FB3, blockIdx.x, blockIdx.y, kb, ke, lds.v);
+    bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
 }
 
 }  // namespace cg
