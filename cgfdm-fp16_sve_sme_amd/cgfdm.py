@@ -91,6 +91,7 @@ def lib():
         L.cgfdm_kernel_timing.argtypes = [vp, i, vp, P(ll)]
         L.cgfdm_enable_timing.argtypes = [vp, i]
         L.cgfdm_partition.argtypes = [vp, vp]
+        L.cgfdm_decompose.argtypes = [vp, vp, i]
         L.cgfdm_set_model_layered.argtypes = [vp, vp, i, vp, vp, vp, vp]
         L.cgfdm_server_create.argtypes = [vp, P(vp)]
         L.cgfdm_server_submit.argtypes = [vp, vp, P(ll)]
@@ -235,6 +236,15 @@ class Domain:
         cnt = ctypes.c_longlong()
         _chk(lib().cgfdm_stage_timing(self.h, 1 if reset else 0, ctypes.byref(avg), ctypes.byref(cnt)))
         return avg.value, cnt.value
+
+
+def decompose(desc):
+    """[(i0, i1, j0, j1, k0, k1, kind, axis)] of the stage decomposition (no device needed)."""
+    out = np.zeros((32, 8), np.int32)
+    n = lib().cgfdm_decompose(ctypes.byref(desc), _p(out), 32)
+    if n < 0:
+        _chk(n)
+    return [tuple(int(x) for x in r) for r in out[:n]]
 
 
 def estimate_bytes(desc):

@@ -822,6 +822,33 @@ int cgfdm_stage_timing(cgfdm_domain* h, int reset, double* avg_ms, long long* co
     return rc;
 }
 
+int cgfdm_decompose(const cgfdm_desc* desc, int* boxes, int max_boxes) {
+    int rc = validate(desc);
+    if (rc) return rc;
+    if (!boxes && max_boxes > 0) { set_err("null box array"); return CGFDM_EINVAL; }
+    StageParams q{};
+    q.nx = desc->NX; q.ny = desc->NY; q.nz = desc->NZ;
+    q.nPML = desc->nPML; q.fs = desc->free_surface; q.pml = desc->pml;
+    stage_boxes(q, 1);
+    int n = 0;
+    auto put = [&](int kind, int axis, const int* B) {
+        if (n < max_boxes) {
+            int* o = boxes + 8 * n;
+            for (int c = 0; c < 6; ++c) o[c] = B[c];
+            o[6] = kind;
+            o[7] = axis;
+        }
+        ++n;
+    };
+    if (q.bulk_on) {
+        const int B[6] = {q.bi0, q.bi1, q.bj0, q.bj1, q.bk0, q.bk1};
+        put(0, -1, B);
+    }
+    for (int b = 0; b < q.npbox; ++b) put(1, q.pbox[b][7], q.pbox[b]);
+    for (int b = 0; b < q.nbox; ++b) put(2, -1, q.box[b]);
+    return n;
+}
+
 int cgfdm_partition(const cgfdm_domain* h, long long* out) {
     if (!h || !out) { set_err("null argument"); return CGFDM_EINVAL; }
     StageParams q = domain_stage_params(h);

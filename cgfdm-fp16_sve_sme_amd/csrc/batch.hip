@@ -13,7 +13,6 @@
 // side bodies), so a domain computes bit-identical results alone and in any mix.
 #include "batch.hpp"
 #include "side.hpp"
-#include "stage_bulk.hpp"
 #include "stage_general.hpp"
 
 namespace cg {
@@ -70,6 +69,7 @@ __global__ void k_clear_slot(BatchDev b, int s) {
 __global__ void k_build_tables(BatchDev b) {
     if (threadIdx.x != 0) return;
     int nb = 0, ng = 0, tiles = 0;
+    int np[3] = {0, 0, 0}, ptiles[3] = {0, 0, 0};
     long long chunks = 0;
     for (int s = 0; s < b.maxSlots; ++s) {
         const SlotDev& sd = b.slots[s];
@@ -84,11 +84,24 @@ __global__ void k_build_tables(BatchDev b) {
             e.slot = s; e.box = x; e.start = chunks;
             chunks += sd.boxChunks[x];
         }
+        for (int a = 0; a < 3; ++a) {
+            const PmlTiling& t = sd.pt[a];
+            for (int x = 0; x < t.n; ++x) {
+                PmlEnt& e = b.tab->pml[a][np[a]++];
+                e.slot = s; e.box = t.box[x]; e.gx = t.gx[x]; e.gy = t.gy[x]; e.nch = t.nch[x]; e.kc = t.kc[x];
+                e.start = ptiles[a];
+                ptiles[a] += t.gx[x] * t.gy[x] * t.nch[x];
+            }
+        }
     }
     b.tab->nBulk = nb;
     b.tab->nTiles = tiles;
     b.tab->nGen = ng;
     b.tab->nChunks = chunks;
+    for (int a = 0; a < 3; ++a) {
+        b.tab->nPml[a] = np[a];
+        b.tab->nPmlTiles[a] = ptiles[a];
+    }
 }
 
 // ---------------------------------------------------------------- per-step kernels
@@ -129,8 +142,28 @@ __global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b) {
         const int bx = r % e.gx, by = (r / e.gx) % e.gy, ch = r / (e.gx * e.gy);
         const int kb = p.bk0 + ch * e.kc;
         const int ke = min(kb + e.kc, p.bk1);
-        bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
+        bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, p.bi0, p.bi1, p.bj0, p.bj1, 0, bx, by, kb, ke, lds.v);
         __syncthreads();  // the next tile rewrites the LDS planes
+    }
+}
+
+template <class S, int TX, int TY, int IRK, int AXIS>
+__global__ void __launch_bounds__(TX* TY) k_pml_multi(BatchDev b) {
+    __shared__ BulkLds<TX, TY> lds;
+    const int nEnt = b.tab->nPml[AXIS], nTiles = b.tab->nPmlTiles[AXIS];
+    for (int w = blockIdx.x; w < nTiles; w += gridDim.x) {
+        const int t = (int)xcd_order(w, nTiles);
+        const PmlEnt& e = b.tab->pml[AXIS][find_entry(b.tab->pml[AXIS], nEnt, t)];
+        const SlotDev& sd = b.slots[e.slot];
+        if (sd.it >= sd.NT) continue;
+        const StageParams& p = b.sp[e.slot * 4 + IRK];
+        const int r = t - e.start;
+        const int bx = r % e.gx, by = (r / e.gx) % e.gy, ch = r / (e.gx * e.gy);
+        const int* B = p.pbox[e.box];
+        const int kb = B[4] + ch * e.kc;
+        const int ke = min(kb + e.kc, B[5]);
+        bulk_tile<S, TX, TY, IRK, AXIS>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[6], bx, by, kb, ke, lds.v);
+        __syncthreads();
     }
 }
 
@@ -193,17 +226,22 @@ int batch_clear_slot(const BatchDev& b, int s, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+template <class S, int TX, int TY, int IRK>
+static void batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+    k_bulk_multi<S, TX, TY, IRK><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
+    k_pml_multi<S, 16, 16, IRK, 0><<<g.pmlBlocks[0], 256, 0, st>>>(b);
+    k_pml_multi<S, 64, 4, IRK, 1><<<g.pmlBlocks[1], 256, 0, st>>>(b);
+    k_pml_multi<S, 64, 4, IRK, 2><<<g.pmlBlocks[2], 256, 0, st>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, IRK);
+}
+
 template <class S, int TX, int TY>
 static int batch_step_t(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
     k_source_multi<S><<<dim3(2, b.maxSlots), 256, 0, st>>>(b);
-    k_bulk_multi<S, TX, TY, 0><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 0);
-    k_bulk_multi<S, TX, TY, 1><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 1);
-    k_bulk_multi<S, TX, TY, 2><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 2);
-    k_bulk_multi<S, TX, TY, 3><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, 3);
+    batch_stage<S, TX, TY, 0>(b, g, st);
+    batch_stage<S, TX, TY, 1>(b, g, st);
+    batch_stage<S, TX, TY, 2>(b, g, st);
+    batch_stage<S, TX, TY, 3>(b, g, st);
     k_record_multi<S><<<dim3(16, b.maxSlots), 256, 0, st>>>(b);
     k_advance<<<(b.maxSlots + 63) / 64, 64, 0, st>>>(b);
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -223,7 +261,7 @@ int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStre
 
 // Persistent grid sizes: as many workgroups as fit on the device at once.
 int batch_grid(int prec, int cfg, BatchGrid& g) {
-    int dev = 0, ncu = 0, nb = 0, ng = 0;
+    int dev = 0, ncu = 0, nb = 0, ng = 0, n0 = 0, n1 = 0, n2 = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -3;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -3;
     hipError_t e1 = hipSuccess, e2 = hipSuccess;
@@ -231,7 +269,10 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
 #define CG_OCC(S)                                                                                                 \
     e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4, 1>, bt, 0)               \
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8, 1>, bt, 0);              \
-    e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);
+    e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);                  \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_pml_multi<S, 16, 16, 1, 0>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_pml_multi<S, 64, 4, 1, 1>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, k_pml_multi<S, 64, 4, 1, 2>, 256, 0);
     switch (prec) {
         case P_F32: CG_OCC(float) break;
         case P_F16: CG_OCC(_Float16) break;
@@ -243,6 +284,8 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     // multiples of the XCD count (see k_bulk_multi)
     g.bulkBlocks = (ncu * (nb > 0 ? nb : 1) + kXcds - 1) / kXcds * kXcds;
     g.genBlocks = (ncu * (ng > 0 ? ng : 1) + kXcds - 1) / kXcds * kXcds;
+    const int npm[3] = {n0, n1, n2};
+    for (int a = 0; a < 3; ++a) g.pmlBlocks[a] = (ncu * (npm[a] > 0 ? npm[a] : 1) + kXcds - 1) / kXcds * kXcds;
     return 0;
 }
 

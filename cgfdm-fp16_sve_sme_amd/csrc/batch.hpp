@@ -1,6 +1,7 @@
 // batch.hpp — device-side tables of the multi-domain executor (batch.hip) and its launchers.
 #pragma once
 #include "common.hpp"
+#include "stage_bulk.hpp"
 
 namespace cg {
 
@@ -14,8 +15,9 @@ struct SlotDev {
     int fs;
     // bulk tiling: gx x gy column tiles, nch chunks of kc planes; general: chunks per box
     int gx, gy, nch, kc;
+    PmlTiling pt[3];    // single-slab PML boxes by normal axis
     int nbox;
-    long long boxChunks[6];
+    long long boxChunks[kMaxGenBoxes];
     Layout lay;
     void* W[4];         // A, B, C, T
     float DT;
@@ -39,13 +41,16 @@ struct SlotDev {
 };
 
 struct BulkEnt { int slot, gx, gy, nch, kc, start; };
+struct PmlEnt { int slot, box, gx, gy, nch, kc, start; };
 struct GenEnt { int slot, box; long long start; };
 
 struct BatchTables {
     int nBulk, nTiles, nGen, pad;
     long long nChunks;
+    int nPml[3], nPmlTiles[3];
     BulkEnt bulk[kMaxSlots];
-    GenEnt gen[kMaxSlots * 6];
+    PmlEnt pml[3][kMaxSlots * 2];
+    GenEnt gen[kMaxSlots * kMaxGenBoxes];
 };
 
 // Device pointers of one batch (passed by value to every batch kernel).
@@ -56,9 +61,10 @@ struct BatchDev {
     int maxSlots;
 };
 
-struct BatchGrid { int bulkBlocks, genBlocks; };
+struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[3]; };
 
 void stage_boxes(StageParams& q, int use_fast);
+PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY);
 int batch_write_slot(const BatchDev& b, int s, const SlotDev& sd, const StageParams& base, hipStream_t st);
 int batch_clear_slot(const BatchDev& b, int s, hipStream_t st);
 int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st);

@@ -105,6 +105,8 @@ struct Slab {
     int n0, n1; // pos = i0 + j0*n0 + k0*n0*n1 with (i0,j0,k0) relative to the slab origin
 };
 
+constexpr int kMaxGenBoxes = 12;
+
 // Everything one RK stage needs. Passed by value to the stage kernels.
 struct StageParams {
     const void* Win;  // stage input wavefield (9 planes)
@@ -123,14 +125,21 @@ struct StageParams {
     int nPML, fs, pml, irk;
     int FB1, FB2, FB3;
     float rDH, DT, Crho;
-    // bulk box [bi0,bi1)x[bj0,bj1)x[bk0,bk1): no PML slab, no free-surface layer; the fast
-    // kernel covers it and the general kernel covers the rest as up to six boxes
+    // Decomposition of the interior (stage_boxes):
+    //  - the bulk box [bi0,bi1)x[bj0,bj1)x[bk0,bk1): no PML slab, no free-surface layer, covered
+    //    by the tile kernel;
+    //  - npbox single-slab PML boxes (the part of a slab outside every other slab and the
+    //    free-surface layers), covered by the tile kernel with that slab's auxiliary update;
+    //  - nbox general boxes, covered point by point: the edges and corners where slabs meet and
+    //    the free-surface layers.
     int bulk_on;
     int bulk_cfg, bulk_kc;  // tile shape index and z-chunk length (0: automatic)
     int bi0, bi1, bj0, bj1, bk0, bk1;
+    int npbox;
+    int pbox[6][8];         // i0,i1,j0,j1,k0,k1, slab index (0..5), axis (0..2)
     int nbox;
-    int box[6][6];          // i0,i1,j0,j1,k0,k1
-    long long boxStart[7];  // prefix sums of box sizes
+    int box[kMaxGenBoxes][6];            // i0,i1,j0,j1,k0,k1
+    long long boxStart[kMaxGenBoxes + 1];  // prefix sums of box sizes
 };
 
 struct SetupParams {

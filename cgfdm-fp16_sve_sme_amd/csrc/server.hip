@@ -248,6 +248,7 @@ int cgfdm_server::setup_resident(Resident& R) {
         sd.kc = std::min(nzb, 16);
         sd.nch = (nzb + sd.kc - 1) / sd.kc;
     }
+    for (int a = 0; a < 3; ++a) sd.pt[a] = pml_tiling(base, a, a == 0 ? 16 : 64, a == 0 ? 16 : 4);
     sd.nbox = base.nbox;
     for (int x = 0; x < base.nbox; ++x)
         sd.boxChunks[x] = (base.boxStart[x + 1] - base.boxStart[x] + kGenChunk - 1) / kGenChunk;

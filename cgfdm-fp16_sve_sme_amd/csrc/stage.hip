@@ -44,26 +44,100 @@ static void add_box(StageParams& q, int i0, int i1, int j0, int j1, int k0, int 
     ++q.nbox;
 }
 
-// Bulk box and the up to six boxes around it that the general kernel covers.
+static void add_pbox(StageParams& q, int i0, int i1, int j0, int j1, int k0, int k1, int slab, int axis) {
+    if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
+    int* B = q.pbox[q.npbox++];
+    B[0] = i0; B[1] = i1; B[2] = j0; B[3] = j1; B[4] = k0; B[5] = k1; B[6] = slab; B[7] = axis;
+}
+
+// Partition of the interior. Along each axis: the low slab [lo, lo+np), the middle and the high
+// slab (z: bottom slab, middle, and either the three free-surface layers or the top slab). The
+// middle^3 cell is the bulk box; a slab cell whose other two ranges are middle is a
+// single-slab PML box; all other cells (edges and corners where slabs meet, the free-surface
+// layers) are general boxes. With use_fast == 0 the whole interior is one general box.
 void stage_boxes(StageParams& q, int use_fast) {
     const int lo = HALO, hiI = HALO + q.nx, hiJ = HALO + q.ny, hiK = HALO + q.nz;
     const int np = q.pml ? q.nPML : 0;
-    q.bi0 = lo + np; q.bi1 = hiI - np;
-    q.bj0 = lo + np; q.bj1 = hiJ - np;
-    q.bk0 = lo + np;
-    q.bk1 = q.fs ? hiK - HALO : hiK - np;
-    q.bulk_on = use_fast && q.bi1 > q.bi0 && q.bj1 > q.bj0 && q.bk1 > q.bk0;
+    const int I0 = lo, I1 = lo + np, I2 = hiI - np;
+    const int J0 = lo, J1 = lo + np, J2 = hiJ - np;
+    const int K0 = lo, K1 = lo + np;
+    const int K2 = q.fs ? hiK - HALO : hiK - np;  // start of the top range
     q.nbox = 0;
+    q.npbox = 0;
     q.boxStart[0] = 0;
-    if (q.bulk_on) {
-        add_box(q, lo, q.bi0, lo, hiJ, lo, hiK);            // x low side, full j, k
-        add_box(q, q.bi1, hiI, lo, hiJ, lo, hiK);           // x high side
-        add_box(q, q.bi0, q.bi1, lo, q.bj0, lo, hiK);       // y low side
-        add_box(q, q.bi0, q.bi1, q.bj1, hiJ, lo, hiK);      // y high side
-        add_box(q, q.bi0, q.bi1, q.bj0, q.bj1, lo, q.bk0);  // bottom
-        add_box(q, q.bi0, q.bi1, q.bj0, q.bj1, q.bk1, hiK); // top (free surface or top PML)
-    } else {
+    q.bi0 = I1; q.bi1 = I2;
+    q.bj0 = J1; q.bj1 = J2;
+    q.bk0 = K1; q.bk1 = K2;
+    q.bulk_on = use_fast && I2 > I1 && J2 > J1 && K2 > K1;
+    if (!use_fast || !(I2 > I1 && J2 > J1 && K2 > K1)) {
+        q.bulk_on = 0;
         add_box(q, lo, hiI, lo, hiJ, lo, hiK);
+        return;
+    }
+    if (np > 0) {
+        add_pbox(q, I0, I1, J1, J2, K1, K2, 0, 0);  // x low
+        add_pbox(q, I2, hiI, J1, J2, K1, K2, 3, 0); // x high
+        add_pbox(q, I1, I2, J0, J1, K1, K2, 1, 1);  // y low
+        add_pbox(q, I1, I2, J2, hiJ, K1, K2, 4, 1); // y high
+        add_pbox(q, I1, I2, J1, J2, K0, K1, 2, 2);  // bottom
+        if (!q.fs) add_pbox(q, I1, I2, J1, J2, K2, hiK, 5, 2);  // top
+    }
+    // bottom ring (z in the bottom slab, outside its single-slab part)
+    add_box(q, lo, hiI, J0, J1, K0, K1);
+    add_box(q, lo, hiI, J2, hiJ, K0, K1);
+    add_box(q, I0, I1, J1, J2, K0, K1);
+    add_box(q, I2, hiI, J1, J2, K0, K1);
+    // vertical edges where the x and y slabs meet
+    add_box(q, I0, I1, J0, J1, K1, K2);
+    add_box(q, I0, I1, J2, hiJ, K1, K2);
+    add_box(q, I2, hiI, J0, J1, K1, K2);
+    add_box(q, I2, hiI, J2, hiJ, K1, K2);
+    if (q.fs) {
+        add_box(q, lo, hiI, lo, hiJ, K2, hiK);  // free-surface layers
+    } else {
+        add_box(q, lo, hiI, J0, J1, K2, hiK);
+        add_box(q, lo, hiI, J2, hiJ, K2, hiK);
+        add_box(q, I0, I1, J1, J2, K2, hiK);
+        add_box(q, I2, hiI, J1, J2, K2, hiK);
+    }
+}
+
+// Tiles of the PML boxes of one normal axis, about `want` workgroups in all.
+PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY) {
+    PmlTiling t{};
+    t.start[0] = 0;
+    int cols = 0;
+    for (int b = 0; b < q.npbox; ++b) {
+        const int* B = q.pbox[b];
+        if (B[7] != axis) continue;
+        const int e = t.n++;
+        t.box[e] = b;
+        t.gx[e] = (B[1] - B[0] + TX - 1) / TX;
+        t.gy[e] = (B[3] - B[2] + TY - 1) / TY;
+        cols += t.gx[e] * t.gy[e];
+    }
+    const int want = 1024;
+    for (int e = 0; e < t.n; ++e) {
+        const int* B = q.pbox[t.box[e]];
+        const int nzb = B[5] - B[4];
+        const int nch = std::max(1, std::min(nzb / 8, (want + cols - 1) / std::max(cols, 1)));
+        t.kc[e] = (nzb + nch - 1) / nch;
+        t.nch[e] = (nzb + t.kc[e] - 1) / t.kc[e];
+        t.start[e + 1] = t.start[e] + t.gx[e] * t.gy[e] * t.nch[e];
+    }
+    return t;
+}
+
+template <class S, int TX, int TY, int AXIS>
+static void launch_pml(const StageParams& q, hipStream_t s) {
+    const PmlTiling t = pml_tiling(q, AXIS, TX, TY);
+    if (t.n == 0 || t.start[t.n] == 0) return;
+    const unsigned grid = (unsigned)t.start[t.n];
+    switch (q.irk) {
+        case 0: k_stage_pml<S, TX, TY, 0, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
+        case 1: k_stage_pml<S, TX, TY, 1, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
+        case 2: k_stage_pml<S, TX, TY, 2, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
+        default: k_stage_pml<S, TX, TY, 3, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
     }
 }
 
@@ -90,6 +164,12 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
         if (hipGetLastError() != hipSuccess) return -3;
     }
     if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;
+    if (use_fast != 2) {
+        launch_pml<S, 16, 16, 0>(q, s);
+        launch_pml<S, 64, 4, 1>(q, s);
+        launch_pml<S, 64, 4, 2>(q, s);
+        if (hipGetLastError() != hipSuccess) return -3;
+    }
     const long long n = q.boxStart[q.nbox];
     if (n > 0 && use_fast != 2) {
         const int bs = 256;

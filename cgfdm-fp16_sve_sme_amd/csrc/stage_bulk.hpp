@@ -32,23 +32,32 @@ template <int TX, int TY> struct BulkLds {
     float v[2][9 * VS];
 };
 
-// One tile: columns [bi0 + bx*TX, +TX) x [bj0 + by*TY, +TY) of the bulk box, planes [kb, ke).
-// Every thread of the workgroup calls it with the same arguments.
-template <class S, int TX, int TY, int IRK>
-__device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int bx, int by, int kb,
-                                          int ke, float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+// One tile: columns [ci0 + bx*TX, +TX) x [cj0 + by*TY, +TY) of the box with columns
+// [ci0,ci1) x [cj0,cj1), planes [kb, ke). Every thread of the workgroup calls it with the same
+// arguments.
+//
+// AXIS < 0: the bulk box, where every PML factor is 1. AXIS = 0, 1, 2: a single-slab PML box of
+// slab `sid` whose normal is that axis (no other slab and no free-surface layer in the box):
+// the derivatives along AXIS are scaled by 1/beta of the point, and the slab's auxiliary
+// variables get their update (pml_deriv.cpp:176-237: -beta*Aux*DT into the increment, the
+// single-direction right-hand side times d minus (alpha+d)*Aux into the auxiliary increment;
+// pml_rk.cpp:14-70: RK4 on the auxiliaries).
+template <class S, int TX, int TY, int IRK, int AXIS = -1>
+__device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int ci0, int ci1, int cj0,
+                                          int cj1, int sid, int bx, int by, int kb, int ke,
+                                          float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
     constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
     constexpr int NT = TX * TY;
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int i0 = p.bi0 + bx * TX, j0 = p.bj0 + by * TY;
+    const int i0 = ci0 + bx * TX, j0 = cj0 + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
     const Layout L = p.lay;
     const long long plane = L.plane;
-    const bool has = i < L.nx_ && j < L.ny_;     // a grid point (possibly outside the box)
-    const bool inside = i < p.bi1 && j < p.bj1;  // computed by this kernel
+    const bool has = i < L.nx_ && j < L.ny_;   // a grid point (possibly outside the box)
+    const bool inside = i < ci1 && j < cj1;    // computed by this kernel
     const unsigned col = has ? (unsigned)L.col(i, j) * (unsigned)sizeof(S) : 0u;  // byte offset in a plane
     const int vs = (int)(plane * (long long)sizeof(S));  // bytes per variable plane
 
@@ -120,6 +129,46 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const float Crho = p.Crho;
     int k = k0;
 
+    // PML slab of the box: per-lane profile values on the x/y axes, aux positions
+    // pos = base + kk*kstride with kk the plane index inside the slab's k range
+    [[maybe_unused]] float pBeta = 1.f, pD = 0.f, pAD = 0.f;
+    [[maybe_unused]] long long pBase = 0, pKs = 0;
+    [[maybe_unused]] int pK0 = 0, pKn = 1;
+    [[maybe_unused]] float *xA = nullptr, *xM = nullptr, *xT = nullptr;
+    [[maybe_unused]] long long xN = 0;
+    [[maybe_unused]] int xn4 = 0;
+    if constexpr (AXIS >= 0) {
+        const Slab& X = p.slab[sid];
+        const int np = p.nPML;
+        if constexpr (AXIS == 0) {
+            pBase = inside ? (long long)(i - X.lo) + (long long)(j - HALO) * np : 0;
+            pKs = (long long)np * p.ny;
+            pK0 = HALO;
+            pKn = p.nz;
+        } else if constexpr (AXIS == 1) {
+            pBase = inside ? (long long)(i - HALO) + (long long)(j - X.lo) * p.nx : 0;
+            pKs = (long long)p.nx * np;
+            pK0 = HALO;
+            pKn = p.nz;
+        } else {
+            pBase = inside ? (long long)(i - HALO) + (long long)(j - HALO) * p.nx : 0;
+            pKs = (long long)p.nx * p.ny;
+            pK0 = X.lo;
+            pKn = np;
+        }
+        if constexpr (AXIS < 2) {
+            const int c = AXIS == 0 ? (inside ? i : HALO) : (inside ? j : HALO);
+            pBeta = p.pb[AXIS][c];
+            pD = p.pd[AXIS][c];
+            pAD = pD + p.pa[AXIS][c];
+        }
+        xA = X.a;
+        xM = X.m;
+        xT = X.t;
+        xN = X.n;
+        xn4 = (int)(X.n * 4);  // bytes per auxiliary variable
+    }
+
     auto step = [&](auto Rc, int it) {
         constexpr int R = decltype(Rc)::value;
         float(&q0)[9] = q[(0 + R) % 5];
@@ -164,6 +213,32 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
             for (int v = 0; v < 9; ++v) rt[v] = bld<S>(rT, col, v * vs);
         }
+        // auxiliaries of this plane (lanes outside the box read element 0)
+        [[maybe_unused]] float xa[9], xm[9], xt[9];
+        [[maybe_unused]] unsigned xoff = 0;
+        [[maybe_unused]] float zBeta = 1.f, zD = 0.f, zAD = 0.f;
+        [[maybe_unused]] const Rsrc rXa = make_rsrc(xA, 9 * xN);
+        [[maybe_unused]] const Rsrc rXm = make_rsrc(xM, 9 * xN);
+        [[maybe_unused]] const Rsrc rXt = make_rsrc(xT, 9 * xN);
+        if constexpr (AXIS >= 0) {
+            const int kk = min(max(kc - pK0, 0), pKn - 1);
+            xoff = inside ? (unsigned)((pBase + (long long)kk * pKs) * 4) : 0u;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) xa[v] = bld<float>(rXa, xoff, v * xn4);
+            if constexpr (IRK == 1 || IRK == 2) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) xm[v] = bld<float>(rXm, xoff, v * xn4);
+            }
+            if constexpr (IRK >= 1) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) xt[v] = bld<float>(rXt, xoff, v * xn4);
+            }
+            if constexpr (AXIS == 2) {
+                zBeta = p.pb[2][kc];
+                zD = p.pd[2][kc];
+                zAD = zD + p.pa[2][kc];
+            }
+        }
         // stage the centre plane and its halo in LDS
 #pragma unroll
         for (int v = 0; v < 9; ++v) Lb[v * VS + lc] = q1[v];
@@ -183,13 +258,17 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             // so only the three derivatives of one variable are live at a time.
             float G[3][3];
             float dv[3] = {0.f, 0.f, 0.f};
+            [[maybe_unused]] float draw[9];  // raw derivative along the slab normal
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const float* Lv = Lb + v * VS;
                 const float c = q1[v];
-                const float d0 = cx0 * Lv[ox0] + cx1 * c + cx2 * Lv[ox2] + cx3 * Lv[ox3] + cx4 * Lv[ox4];
-                const float d1 = cy0 * Lv[oy0] + cy1 * c + cy2 * Lv[oy2] + cy3 * Lv[oy3] + cy4 * Lv[oy4];
-                const float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
+                float d0 = cx0 * Lv[ox0] + cx1 * c + cx2 * Lv[ox2] + cx3 * Lv[ox3] + cx4 * Lv[ox4];
+                float d1 = cy0 * Lv[oy0] + cy1 * c + cy2 * Lv[oy2] + cy3 * Lv[oy3] + cy4 * Lv[oy4];
+                float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
+                if constexpr (AXIS == 0) { draw[v] = d0; d0 *= pBeta; }
+                if constexpr (AXIS == 1) { draw[v] = d1; d1 *= pBeta; }
+                if constexpr (AXIS == 2) { draw[v] = d2; d2 *= zBeta; }
                 if (v < 3) {
 #pragma unroll
                     for (int cc = 0; cc < 3; ++cc) G[v][cc] = M[cc] * d0 + M[3 + cc] * d1 + M[6 + cc] * d2;
@@ -214,6 +293,47 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             h[6] = mu * (G[0][1] + G[1][0]);
             h[7] = mu * (G[0][2] + G[2][0]);
             h[8] = mu * (G[1][2] + G[2][1]);
+            if constexpr (AXIS >= 0) {
+                const float beta = AXIS == 2 ? zBeta : pBeta;
+                const float dd = AXIS == 2 ? zD : pD;
+                const float ad = AXIS == 2 ? zAD : pAD;
+                // -beta*Aux*DT into the increment, then the single-direction right-hand side
+                // (with d, no beta) of the slab normal
+#pragma unroll
+                for (int v = 0; v < 9; ++v) h[v] += -beta * xa[v] * DT;
+                const float mx = M[3 * AXIS], my = M[3 * AXIS + 1], mz = M[3 * AXIS + 2];
+                float b[9];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) b[v] = draw[v] * dd;
+                float ha[9];
+                ha[0] = (mx * b[3] + my * b[6] + mz * b[7]) * bu;
+                ha[1] = (mx * b[6] + my * b[4] + mz * b[8]) * bu;
+                ha[2] = (mx * b[7] + my * b[8] + mz * b[5]) * bu;
+                const float pdv = (mx * b[0] + my * b[1] + mz * b[2]) * lam;
+                ha[3] = pdv + mu2 * (mx * b[0]);
+                ha[4] = pdv + mu2 * (my * b[1]);
+                ha[5] = pdv + mu2 * (mz * b[2]);
+                ha[6] = (my * b[0] + mx * b[1]) * mu;
+                ha[7] = (mz * b[0] + mx * b[2]) * mu;
+                ha[8] = (mz * b[1] + my * b[2]) * mu;
+#pragma unroll
+                for (int v = 0; v < 9; ++v) {
+                    const float hav = ha[v] - ad * xa[v] * DT;
+                    if constexpr (IRK == 0) {
+                        bst<float>(rXm, xoff, v * xn4, xa[v]);
+                        bst<float>(rXt, xoff, v * xn4, xa[v] + beta1 * hav);
+                        bst<float>(rXa, xoff, v * xn4, xa[v] + alpha2 * hav);
+                    } else if constexpr (IRK == 1) {
+                        bst<float>(rXt, xoff, v * xn4, xt[v] + beta2 * hav);
+                        bst<float>(rXa, xoff, v * xn4, xm[v] + alpha3 * hav);
+                    } else if constexpr (IRK == 2) {
+                        bst<float>(rXt, xoff, v * xn4, xt[v] + beta3 * hav);
+                        bst<float>(rXa, xoff, v * xn4, xm[v] + hav);
+                    } else {
+                        bst<float>(rXa, xoff, v * xn4, xt[v] + beta4 * hav);
+                    }
+                }
+            }
             // RK registers (wave_rk0..3)
             const Rsrc rO = make_rsrc((const S*)(Ob + pw), wrange);
 #pragma unroll
@@ -262,7 +382,30 @@ __global__ void __launch_bounds__(TX* TY) k_stage_bulk(StageParams p, int kc, in
     const int bx = (int)(t % gx), by = (int)((t / gx) % gy), bz = (int)(t / ((long long)gx * gy));
     const int kb = p.bk0 + bz * kc;
     const int ke = min(kb + kc, p.bk1);
-    bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, bx, by, kb, ke, lds.v);
+    bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, p.bi0, p.bi1, p.bj0, p.bj1, 0, bx, by, kb, ke, lds.v);
+}
+
+// Tiling of the single-slab PML boxes of one normal axis (one launch per axis): box e of the
+// list is p.pbox[box[e]], cut into gx*gy column tiles and nch chunks of kc planes; tile indices
+// are consecutive per box (start[e]).
+struct PmlTiling {
+    int n;
+    int box[2];
+    int gx[2], gy[2], nch[2], kc[2];
+    int start[3];
+};
+
+template <class S, int TX, int TY, int IRK, int AXIS>
+__global__ void __launch_bounds__(TX* TY) k_stage_pml(StageParams p, PmlTiling tl) {
+    __shared__ BulkLds<TX, TY> lds;
+    const int t = (int)xcd_order(blockIdx.x, (long long)gridDim.x);
+    const int e = (tl.n > 1 && t >= tl.start[1]) ? 1 : 0;
+    const int r = t - tl.start[e];
+    const int bx = r % tl.gx[e], by = (r / tl.gx[e]) % tl.gy[e], bz = r / (tl.gx[e] * tl.gy[e]);
+    const int* B = p.pbox[tl.box[e]];
+    const int kb = B[4] + bz * tl.kc[e];
+    const int ke = min(kb + tl.kc[e], B[5]);
+    bulk_tile<S, TX, TY, IRK, AXIS>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[6], bx, by, kb, ke, lds.v);
 }
 
 }  // namespace cg

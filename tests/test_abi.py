@@ -83,3 +83,37 @@ def test_no_fallback_when_library_missing(gpu, monkeypatch):
     monkeypatch.setattr(cgfdm, "_lib", None)
     with pytest.raises(RuntimeError):
         cgfdm.lib()
+
+
+@pytest.mark.parametrize("fs,pml,npml,shape", [(1, 1, 12, (60, 50, 40)), (0, 1, 6, (30, 26, 20)),
+                                               (1, 0, 6, (20, 18, 12)), (0, 0, 5, (16, 16, 16)),
+                                               (1, 1, 8, (18, 40, 30)), (1, 1, 4, (9, 9, 10))])
+def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
+    """Bulk box, single-slab PML boxes and general boxes tile the interior exactly once; a PML
+    box lies inside exactly one slab (its normal axis) and outside the free-surface layers."""
+    import numpy as np
+    NX, NY, NZ = shape
+    d = gpu.make_desc(NX, NY, NZ, nPML=npml, free_surface=fs, pml=pml)
+    boxes = gpu.decompose(d)
+    cover = np.zeros((NZ + 6, NY + 6, NX + 6), np.int32)
+    for i0, i1, j0, j1, k0, k1, kind, axis in boxes:
+        assert i0 < i1 and j0 < j1 and k0 < k1
+        cover[k0:k1, j0:j1, i0:i1] += 1
+        if kind == 1:
+            ranges = [(i0, i1, NX), (j0, j1, NY), (k0, k1, NZ)]
+            for a, (b0, b1, n) in enumerate(ranges):
+                in_low = b1 <= 3 + npml
+                in_high = b0 >= 3 + n - npml
+                top = 3 + n - (3 if (a == 2 and fs) else npml)  # free-surface layers or top slab
+                if a == axis:
+                    assert in_low or (in_high and not (a == 2 and fs))
+                else:
+                    assert b0 >= 3 + npml and b1 <= top
+            if fs:
+                assert k1 <= 3 + NZ - 3
+    inner = cover[3:NZ + 3, 3:NY + 3, 3:NX + 3]
+    assert inner.min() == 1 and inner.max() == 1
+    assert cover.sum() == NX * NY * NZ
+    kinds = [b[6] for b in boxes]
+    if pml and min(NX, NY, NZ) > 2 * npml + 3:
+        assert kinds.count(0) == 1 and kinds.count(1) == (5 if fs else 6)
