@@ -149,15 +149,15 @@ int validate(const cgfdm_desc* d) {
     return 0;
 }
 
-// Internal layout: rows padded to a multiple of 32 elements and shifted so that the first column of
-// the bulk box (3 + nPML) is 64-byte aligned for both storage widths.
+// Internal layout (Layout in common.hpp): rows padded to whole 64-column blocks and shifted so
+// that the first column of the bulk box (3 + nPML) starts a block.
 Layout make_layout(const cgfdm_desc& d) {
     Layout L{};
     L.nx_ = d.NX + 6; L.ny_ = d.NY + 6; L.nz_ = d.NZ + 6;
-    const int align = 32;
     const int bi0 = HALO + (d.pml ? d.nPML : 0);
-    L.lead = (align - bi0 % align) % align;
-    L.px = (L.lead + L.nx_ + align - 1) / align * align;
+    L.lead = (kBlk - bi0 % kBlk) % kBlk;
+    L.px = (L.lead + L.nx_ + kBlk - 1) / kBlk * kBlk;
+    L.nb = L.px / kBlk;
     L.plane = (long long)L.px * L.ny_;
     return L;
 }

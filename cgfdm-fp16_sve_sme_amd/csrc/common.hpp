@@ -30,18 +30,25 @@ template <> struct SV<__bf16> {
     __device__ __forceinline__ static __bf16 st(float v) { return (__bf16)v; }
 };
 
-// Internal memory layout of every 3-D array: k-major with the variables of one z-plane stored
-// consecutively, [k][var][j][i], rows padded to `px` elements and shifted right by `lead` so that
-// the first column of the bulk box starts on a 64-byte boundary. A plane of one variable is
-// `plane` = px * ny_ elements. For a given (k, var) the wave-uniform part of an address is
-// (k*V + var)*plane and the per-lane part is the 32-bit column offset j*px + i + lead.
+// Internal memory layout of every 3-D array with V variables: block-interleaved,
+// [k][j][x block][var][64]. Rows are padded to px = nb*64 elements and shifted right by `lead` so
+// that the first column of the bulk box starts a block; a block holds the 64 values of variable
+// 0, then the 64 of variable 1, and so on. A wave that reads every variable of 64 consecutive
+// points therefore reads one contiguous run of V*64 elements (instead of V streams one variable
+// plane apart, which the memory system served ~25 % slower in a bandwidth probe of this access
+// mix), and a k-plane of all variables is one contiguous region of V*plane elements
+// (plane = px*ny_). col() is the offset of (i, j) inside a k-plane; the variable adds v*64.
+constexpr int kBlk = 64;
 struct Layout {
     int nx_, ny_, nz_;  // padded sizes (interior + 2*HALO)
-    int px, lead;       // row pitch and column shift, in elements
+    int px, lead, nb;   // row pitch, column shift (elements), blocks per row
     long long plane;    // px * ny_
-    __host__ __device__ __forceinline__ long long col(int i, int j) const { return (long long)j * px + i + lead; }
+    __host__ __device__ __forceinline__ long long col(int V, int i, int j) const {
+        const int ii = i + lead;
+        return ((long long)j * nb + (ii >> 6)) * V * kBlk + (ii & (kBlk - 1));
+    }
     __host__ __device__ __forceinline__ long long at(int V, int v, int i, int j, int k) const {
-        return ((long long)k * V + v) * plane + col(i, j);
+        return (long long)k * V * plane + col(V, i, j) + (long long)v * kBlk;
     }
     __host__ __device__ __forceinline__ long long size(int V) const { return (long long)nz_ * V * plane; }
 };

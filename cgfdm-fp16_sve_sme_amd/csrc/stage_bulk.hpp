@@ -11,9 +11,10 @@
 //    12 metric/medium values and the RK registers) is unconditional and issued at the top of the
 //    iteration, before any wait: no load sits in a branch, where the compiler would have to wait
 //    for it inside the branch and serialise the iteration's memory latency;
-//  - global accesses are buffer loads/stores: one descriptor per array and plane (k-major
-//    [k][var][j][i] layout), a per-lane byte offset that does not change along the march and a
-//    wave-uniform per-variable offset, so no VALU address arithmetic is spent;
+//  - global accesses are buffer loads/stores: one descriptor per array and plane (block-
+//    interleaved [k][j][x block][var][64] layout, so the variables of a wave's 64 points are one
+//    contiguous run), a per-lane byte offset that does not change along the march and a constant
+//    per-variable offset, so no VALU address arithmetic is spent;
 //  - the RK stage is a template parameter, so the register update has no run-time branches;
 //  - the z loop is unrolled by 5 (the queue rotates by renaming) and always runs a multiple of 5
 //    iterations: the ones past the chunk load clamped planes and store nothing, so there is a
@@ -58,8 +59,11 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const long long plane = L.plane;
     const bool has = i < L.nx_ && j < L.ny_;   // a grid point (possibly outside the box)
     const bool inside = i < ci1 && j < cj1;    // computed by this kernel
-    const unsigned col = has ? (unsigned)L.col(i, j) * (unsigned)sizeof(S) : 0u;  // byte offset in a plane
-    const int vs = (int)(plane * (long long)sizeof(S));  // bytes per variable plane
+    // byte offsets of the point in a k-plane of a 9-variable array (W, A, T, Wout) and of the
+    // 13-component metric/medium array; a variable adds vs bytes
+    const unsigned col = has ? (unsigned)L.col(9, i, j) * (unsigned)sizeof(S) : 0u;
+    const unsigned colc = has ? (unsigned)L.col(13, i, j) * (unsigned)sizeof(S) : 0u;
+    constexpr int vs = kBlk * (int)sizeof(S);
 
     // z chunk, traversed in the direction FB3
     const int nk = ke - kb;
@@ -120,7 +124,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             }
             const int ig = i0 - hx + hc, jg = j0 - hy + hr;
             const bool hhas = ig >= 0 && ig < L.nx_ && jg >= 0 && jg < L.ny_;
-            hcol[r] = hhas ? (unsigned)L.col(ig, jg) * (unsigned)sizeof(S) : 0u;
+            hcol[r] = hhas ? (unsigned)L.col(9, ig, jg) * (unsigned)sizeof(S) : 0u;
             hl[r] = hr * LX + hc;
         }
     }
@@ -198,9 +202,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         {
             const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kc * cpl), crange);
 #pragma unroll
-            for (int c = 0; c < 9; ++c) M[c] = bld<S>(rC, col, c * vs);
+            for (int c = 0; c < 9; ++c) M[c] = bld<S>(rC, colc, c * vs);
 #pragma unroll
-            for (int c = 10; c < 13; ++c) M[c - 1] = bld<S>(rC, col, c * vs);
+            for (int c = 10; c < 13; ++c) M[c - 1] = bld<S>(rC, colc, c * vs);
         }
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         float ra[9], rt[9];

@@ -14,14 +14,14 @@
 
 namespace cg {
 
+// One-sided derivative of variable v along direction d at (i, j, k):
+// FB*(a_-1 f[-FB] + a0 f[0] + a1 f[FB] + a2 f[2FB] + a3 f[3FB])*rDH  (axis.h:129-133)
 template <class S>
-__device__ __forceinline__ float Lraw(const S* __restrict__ w, long long idx, long long st, int FB, float rDH) {
-    // FB*(a_-1 f[-FB] + a0 f[0] + a1 f[FB] + a2 f[2FB] + a3 f[3FB])*rDH  (axis.h:129-133)
-    const long long s = FB * st;
-    return (float)FB *
-           (af_1 * SV<S>::ld(w[idx - s]) + af0 * SV<S>::ld(w[idx]) + af1 * SV<S>::ld(w[idx + s]) +
-            af2 * SV<S>::ld(w[idx + 2 * s]) + af3 * SV<S>::ld(w[idx + 3 * s])) *
-           rDH;
+__device__ __forceinline__ float Lraw(const S* __restrict__ W, const Layout& L, int v, int i, int j, int k, int d,
+                                      int FB, float rDH) {
+    const int di = d == 0 ? FB : 0, dj = d == 1 ? FB : 0, dk = d == 2 ? FB : 0;
+    auto f = [&](int m) { return SV<S>::ld(W[L.at(9, v, i + m * di, j + m * dj, k + m * dk)]); };
+    return (float)FB * (af_1 * f(-1) + af0 * f(0) + af1 * f(1) + af2 * f(2) + af3 * f(3)) * rDH;
 }
 
 __device__ __forceinline__ float LJT(const float* J_T, int FB, float rDH) {
@@ -29,16 +29,16 @@ __device__ __forceinline__ float LJT(const float* J_T, int FB, float rDH) {
            (af_1 * J_T[3 - FB] + af0 * J_T[3] + af1 * J_T[3 + FB] + af2 * J_T[3 + 2 * FB] + af3 * J_T[3 + 3 * FB]) * rDH;
 }
 
-// RK register update of the 9 wavefield variables at element idx of variable 0 (wave_rk0..3).
+// RK register update of the 9 wavefield variables at (i, j, k) (wave_rk0..3).
 template <class S>
-__device__ __forceinline__ void rk_update(const StageParams& p, long long idx, const float* h) {
+__device__ __forceinline__ void rk_update(const StageParams& p, int i, int j, int k, const float* h) {
     S* __restrict__ Wo = (S*)p.Wout;
     const S* __restrict__ A = (const S*)p.A;
     S* __restrict__ T = (S*)p.T;
-    const long long plane = p.lay.plane;
+    const long long q0 = p.lay.at(9, 0, i, j, k);
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-        const long long q = v * plane + idx;
+        const long long q = q0 + v * kBlk;
         if (p.irk == 0) {
             float m = SV<S>::ld(A[q]);
             Wo[q] = SV<S>::st(m + alpha2 * h[v]);
@@ -62,33 +62,27 @@ template <class S>
 __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
     const S* __restrict__ W = (const S*)p.Win;
     const S* __restrict__ Cj = (const S*)p.cjm;
-    // W element (v,i,j,k) = v*plane + idx (z stride 9*plane); CJM (c,i,j,k) = c*plane + idc
-    const long long plane = p.lay.plane;
-    const long long sy = p.lay.px;
-    const long long szW = 9 * plane, szC = 13 * plane;
-    const long long colij = p.lay.col(i, j);
-    const long long idx = colij + k * szW;
-    const long long idc = colij + k * szC;
+    const Layout& L = p.lay;
+    auto w = [&](int v, int ii, int jj, int kk) { return SV<S>::ld(W[L.at(9, v, ii, jj, kk)]); };
+    auto cj = [&](int c, int ii, int jj, int kk) { return SV<S>::ld(Cj[L.at(13, c, ii, jj, kk)]); };
     const long long rdzi = (i + (long long)j * p.nx_) * 9;
     const float rDH = p.rDH, DT = p.DT;
     const int _nz = p.nz + HALO;
     const int FBs[3] = {p.FB1, p.FB2, p.FB3};
-    const long long sW[3] = {1, sy, szW};
-    const long long sC[3] = {1, sy, szC};
 
     float M[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) M[c] = SV<S>::ld(Cj[c * plane + idc]);
-    const float mu = SV<S>::ld(Cj[10 * plane + idc]);
-    const float lambda = SV<S>::ld(Cj[11 * plane + idc]);
-    const float buoyancy = SV<S>::ld(Cj[12 * plane + idc]) * p.Crho;
+    for (int c = 0; c < 9; ++c) M[c] = cj(c, i, j, k);
+    const float mu = cj(10, i, j, k);
+    const float lambda = cj(11, i, j, k);
+    const float buoyancy = cj(12, i, j, k) * p.Crho;
 
     // raw one-sided derivatives D[dir][var] (no PML factor)
     float D[3][9];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int v = 0; v < 9; ++v) D[d][v] = Lraw<S>(W + v * plane, idx, sW[d], FBs[d], rDH);
+        for (int v = 0; v < 9; ++v) D[d][v] = Lraw<S>(W, L, v, i, j, k, d, FBs[d], rDH);
 
     const float bx = p.pb[0][i], by = p.pb[1][j], bz = p.pb[2][k];
     float h[9];
@@ -136,30 +130,28 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
         for (int l = 0; l <= 2 * HALO; ++l) {
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
-                const long long pos = idx + (l - HALO) * sW[d];
-                const long long pc = idc + (l - HALO) * sC[d];
-                const float m0 = SV<S>::ld(Cj[(3 * d) * plane + pc]);
-                const float m1 = SV<S>::ld(Cj[(3 * d + 1) * plane + pc]);
-                const float m2 = SV<S>::ld(Cj[(3 * d + 2) * plane + pc]);
-                const float jac = SV<S>::ld(Cj[9 * plane + pc]);
-                const float w3 = SV<S>::ld(W[3 * plane + pos]), w4 = SV<S>::ld(W[4 * plane + pos]);
-                const float w5 = SV<S>::ld(W[5 * plane + pos]), w6 = SV<S>::ld(W[6 * plane + pos]);
-                const float w7 = SV<S>::ld(W[7 * plane + pos]), w8 = SV<S>::ld(W[8 * plane + pos]);
+                const int ii = i + (d == 0 ? l - HALO : 0), jj = j + (d == 1 ? l - HALO : 0);
+                const float m0 = cj(3 * d, ii, jj, k);
+                const float m1 = cj(3 * d + 1, ii, jj, k);
+                const float m2 = cj(3 * d + 2, ii, jj, k);
+                const float jac = cj(9, ii, jj, k);
+                const float w3 = w(3, ii, jj, k), w4 = w(4, ii, jj, k);
+                const float w5 = w(5, ii, jj, k), w6 = w(6, ii, jj, k);
+                const float w7 = w(7, ii, jj, k), w8 = w(8, ii, jj, k);
                 JT[d][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
                 JT[d][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
                 JT[d][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
             }
         }
         for (int l = 0; l < k_s; ++l) {
-            const long long pos = idx + (l - HALO) * szW;
-            const long long pc = idc + (l - HALO) * szC;
-            const float m0 = SV<S>::ld(Cj[6 * plane + pc]);
-            const float m1 = SV<S>::ld(Cj[7 * plane + pc]);
-            const float m2 = SV<S>::ld(Cj[8 * plane + pc]);
-            const float jac = SV<S>::ld(Cj[9 * plane + pc]);
-            const float w3 = SV<S>::ld(W[3 * plane + pos]), w4 = SV<S>::ld(W[4 * plane + pos]);
-            const float w5 = SV<S>::ld(W[5 * plane + pos]), w6 = SV<S>::ld(W[6 * plane + pos]);
-            const float w7 = SV<S>::ld(W[7 * plane + pos]), w8 = SV<S>::ld(W[8 * plane + pos]);
+            const int kk = k + l - HALO;
+            const float m0 = cj(6, i, j, kk);
+            const float m1 = cj(7, i, j, kk);
+            const float m2 = cj(8, i, j, kk);
+            const float jac = cj(9, i, j, kk);
+            const float w3 = w(3, i, j, kk), w4 = w(4, i, j, kk);
+            const float w5 = w(5, i, j, kk), w6 = w(6, i, j, kk);
+            const float w7 = w(7, i, j, kk), w8 = w(8, i, j, kk);
             JT[2][0][l] = (m0 * w3 + m1 * w6 + m2 * w7) * jac;
             JT[2][1][l] = (m0 * w6 + m1 * w4 + m2 * w8) * jac;
             JT[2][2][l] = (m0 * w7 + m1 * w8 + m2 * w5) * jac;
@@ -168,7 +160,7 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             JT[2][c][k_s] = 0.f;
             for (int l = k_s + 1; l <= 2 * HALO; ++l) JT[2][c][l] = -JT[2][c][2 * k_s - l];
         }
-        const float jacb = SV<S>::ld(Cj[9 * plane + idc]);
+        const float jacb = cj(9, i, j, k);
         const float Jinv = 1.0f / jacb;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -186,19 +178,16 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             Vy_zt = (X[3] * Vx_xi + X[4] * Vy_xi + X[5] * Vz_xi) + (Y[3] * Vx_et + Y[4] * Vy_et + Y[5] * Vz_et);
             Vz_zt = (X[6] * Vx_xi + X[7] * Vy_xi + X[8] * Vz_xi) + (Y[6] * Vx_et + Y[7] * Vy_et + Y[8] * Vz_et);
         } else if (k == _nz - 2) {  // L2 (axis.h:92)
-            const long long s = p.FB3 * szW;
-            Vx_zt = (float)p.FB3 * (SV<S>::ld(W[0 * plane + idx + s]) - SV<S>::ld(W[0 * plane + idx])) * rDH;
-            Vy_zt = (float)p.FB3 * (SV<S>::ld(W[1 * plane + idx + s]) - SV<S>::ld(W[1 * plane + idx])) * rDH;
-            Vz_zt = (float)p.FB3 * (SV<S>::ld(W[2 * plane + idx + s]) - SV<S>::ld(W[2 * plane + idx])) * rDH;
+            const int k1 = k + p.FB3;
+            Vx_zt = (float)p.FB3 * (w(0, i, j, k1) - w(0, i, j, k)) * rDH;
+            Vy_zt = (float)p.FB3 * (w(1, i, j, k1) - w(1, i, j, k)) * rDH;
+            Vz_zt = (float)p.FB3 * (w(2, i, j, k1) - w(2, i, j, k)) * rDH;
         } else {  // L3 (axis.h:93)
-            const long long s = p.FB3 * szW;
+            const int k1 = k + p.FB3, k2 = k + 2 * p.FB3;
             float r3[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v)
-                r3[v] = (float)p.FB3 *
-                        (Cf1 * SV<S>::ld(W[v * plane + idx]) + Cf2 * SV<S>::ld(W[v * plane + idx + s]) +
-                         Cf3 * SV<S>::ld(W[v * plane + idx + 2 * s])) *
-                        rDH;
+                r3[v] = (float)p.FB3 * (Cf1 * w(v, i, j, k) + Cf2 * w(v, i, j, k1) + Cf3 * w(v, i, j, k2)) * rDH;
             Vx_zt = r3[0]; Vy_zt = r3[1]; Vz_zt = r3[2];
         }
         const float Vd[3][3] = {{Vx_xi, Vy_xi, Vz_xi}, {Vx_et, Vy_et, Vz_et}, {Vx_zt, Vy_zt, Vz_zt}};
@@ -293,7 +282,7 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             }
         }
     }
-    rk_update<S>(p, idx, h);
+    rk_update<S>(p, i, j, k, h);
 }
 
 }  // namespace cg
