@@ -203,7 +203,8 @@ def serving(n, seed, max_domains, steps_per_launch):
     out["requests"] = n
     out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at 20/s, "
                      f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
-                     f"couples 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface stations")
+                     f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface "
+                     f"stations")
     out["speedup"] = round(out["batched"]["solves_per_s"] / out["one_at_a_time"]["solves_per_s"], 3)
     return out
 

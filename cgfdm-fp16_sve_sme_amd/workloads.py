@@ -181,11 +181,13 @@ def layer_mu_at(depth_m, layers):
 
 def serving_stream(n, seed, rate_hz=20.0, edge_xy=(101, 301), edge_z=(51, 151), layers=(2, 5),
                    vs_range=(1500.0, 4000.0), depth_km=(1.0, 15.0), fc=(0.5, 3.0), steps=(500, 1500),
-                   DH=100.0, DT=0.008, nPML=12, precision=2, n_st=5):
+                   DH=100.0, DT=0.008, nPML=12, precision=2, n_st=5, M0=1e15):
     """Seeded Green's-function-style request stream: Poisson arrivals at rate_hz, grid edges
     uniform in edge_xy (x, y) and edge_z (z), random layered media, random double couples at a
     random depth (kept inside the grid above the bottom PML), Ricker moment rate of random peak
-    frequency, random step counts, an n_st x n_st surface station grid. Returns a list of
+    frequency, random step counts, an n_st x n_st surface station grid. M0 = 1e15 N m (Mw 4):
+    with the reference's FP16 scales (Cv = 5e3, Cs = 1e-6) a point source of 1e17 N m in one
+    100 m cell drives the velocity at that cell past the FP16 range (|V| < 13 m/s). Returns a list of
     (arrival_s, request) with request dicts as cgfdm.Server.submit takes them (desc is built
     by the caller from request['grid'] etc. to avoid importing the binding here)."""
     rng = np.random.default_rng(seed)
@@ -207,7 +209,7 @@ def serving_stream(n, seed, rate_hz=20.0, edge_xy=(101, 301), edge_z=(51, 151), 
         X = int(rng.integers(NX // 4, 3 * NX // 4))
         Y = int(rng.integers(NY // 4, 3 * NY // 4))
         mu = layer_mu_at((NZ - 1 - Z) * DH, lay)
-        table = double_couple_table(1e17, strike, dip, rake, f, NT + 2, DT, mu)
+        table = double_couple_table(M0, strike, dip, rake, f, NT + 2, DT, mu)
         st = surface_stations(NX, NY, NZ, n=n_st)
         out.append((t, {"grid": (NX, NY, NZ), "DH": DH, "DT": DT, "NT": NT, "nPML": nPML,
                         "precision": precision, "layers": lay,
