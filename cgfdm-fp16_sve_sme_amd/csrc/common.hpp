@@ -30,27 +30,50 @@ template <> struct SV<__bf16> {
     __device__ __forceinline__ static __bf16 st(float v) { return (__bf16)v; }
 };
 
-// Internal memory layout of every 3-D array with V variables: block-interleaved,
-// [k][j][x block][var][64]. Rows are padded to px = nb*64 elements and shifted right by `lead` so
-// that the first column of the bulk box starts a block; a block holds the 64 values of variable
-// 0, then the 64 of variable 1, and so on. A wave that reads every variable of 64 consecutive
-// points therefore reads one contiguous run of V*64 elements (instead of V streams one variable
-// plane apart, which the memory system served ~25 % slower in a bandwidth probe of this access
-// mix), and a k-plane of all variables is one contiguous region of V*plane elements
-// (plane = px*ny_). col() is the offset of (i, j) inside a k-plane; the variable adds v*64.
+// Internal memory layout of every 3-D array with V variables: block-interleaved by variable
+// pairs, [k][j][x block][...][64]. Rows are padded to px = nb*64 elements and shifted right by
+// `lead` so that the first column of the bulk box starts a block. A block (V*64 elements) holds,
+// for each pair of variables (2p, 2p+1), the 64 points' values interleaved point by point
+// ([64][2]), followed, for odd V, by the 64 values of the last variable alone. One 32-bit (16-bit
+// storage) or 64-bit (FP32 storage) load per lane thus brings both variables of a pair, a wave's
+// loads of all variables of 64 consecutive points cover one contiguous run with no unused
+// element, and a k-plane of all variables is one contiguous region of V*plane elements
+// (plane = px*ny_). The bandwidth probe of this access mix served contiguous per-block runs
+// ~25 % faster than one stream per variable plane.
+//
+// The metric/medium array (13 components) stores the reference components in the slot order
+// 0-8 metrics, 9 mu, 10 lambda, 11 buoyancy, 12 J, so that the 12 values a stage reads (all but
+// J) are exactly the six pairs; atc() takes the reference numbering (0-8 metrics, 9 J, 10 mu,
+// 11 lambda, 12 buoyancy).
 constexpr int kBlk = 64;
+__host__ __device__ __forceinline__ constexpr int cjm_slot(int c) { return c < 9 ? c : (c == 9 ? 12 : c - 1); }
 struct Layout {
     int nx_, ny_, nz_;  // padded sizes (interior + 2*HALO)
     int px, lead, nb;   // row pitch, column shift (elements), blocks per row
     long long plane;    // px * ny_
-    __host__ __device__ __forceinline__ long long col(int V, int i, int j) const {
-        const int ii = i + lead;
-        return ((long long)j * nb + (ii >> 6)) * V * kBlk + (ii & (kBlk - 1));
+    // offset of the block holding (i, j) in a k-plane of a V-variable array
+    __host__ __device__ __forceinline__ long long blk(int V, int i, int j) const {
+        return ((long long)j * nb + ((i + lead) >> 6)) * V * kBlk;
     }
+    __host__ __device__ __forceinline__ int lane(int i) const { return (i + lead) & (kBlk - 1); }
+    // offset of variable (slot) v of the point in lane ln, relative to its block
+    __host__ __device__ __forceinline__ static long long var_off(int V, int v, int ln) {
+        return v < (V & ~1) ? (long long)(v >> 1) * 2 * kBlk + 2 * ln + (v & 1) : (long long)(V - 1) * kBlk + ln;
+    }
+    // offsets inside a k-plane of (i, j): its first pair, and the unpaired last variable (odd V)
+    __host__ __device__ __forceinline__ long long col(int V, int i, int j) const { return blk(V, i, j) + 2 * lane(i); }
+    __host__ __device__ __forceinline__ long long col1(int V, int i, int j) const {
+        return blk(V, i, j) + (long long)(V - 1) * kBlk + lane(i);
+    }
+    __host__ __device__ __forceinline__ long long kplane(int V) const { return (long long)V * plane; }
     __host__ __device__ __forceinline__ long long at(int V, int v, int i, int j, int k) const {
-        return (long long)k * V * plane + col(V, i, j) + (long long)v * kBlk;
+        return (long long)k * kplane(V) + blk(V, i, j) + var_off(V, v, lane(i));
     }
-    __host__ __device__ __forceinline__ long long size(int V) const { return (long long)nz_ * V * plane; }
+    // metric/medium array, reference component numbering
+    __host__ __device__ __forceinline__ long long atc(int c, int i, int j, int k) const {
+        return at(13, cjm_slot(c), i, j, k);
+    }
+    __host__ __device__ __forceinline__ long long size(int V) const { return (long long)nz_ * kplane(V); }
 };
 
 // Global access with a wave-uniform base pointer and a 32-bit byte offset per lane, so the load

@@ -44,8 +44,8 @@ __global__ void k_scale_mrate(const S* __restrict__ cjm, Layout L, const int* __
     if (t >= npts * nt) return;
     const long long p = t % npts;
     const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
-    const float jacb = SV<S>::ld(cjm[L.at(13, 9, i, j, k)]);
-    const float mu = SV<S>::ld(cjm[L.at(13, 10, i, j, k)]);
+    const float jacb = SV<S>::ld(cjm[L.atc(9, i, j, k)]);
+    const float mu = SV<S>::ld(cjm[L.atc(10, i, j, k)]);
     float V = jacb * DH * DH * DH;
     V = 1.0f / V;
     V = V * Cv;
@@ -106,6 +106,7 @@ __global__ void k_scatter(S* A, Layout L, int V, int nx, int ny, int nz, const f
 }
 
 // Whole padded array to the reference order (V, nz_, ny_, nx_).
+// V == 13 is the metric/medium array: out in the reference component order.
 template <class S>
 __global__ void k_unpack(const S* __restrict__ A, Layout L, int V, float* __restrict__ out) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -114,7 +115,7 @@ __global__ void k_unpack(const S* __restrict__ A, Layout L, int V, float* __rest
     const int v = (int)(t / n);
     const long long r = t % n;
     const int i = (int)(r % L.nx_), j = (int)((r / L.nx_) % L.ny_), k = (int)(r / ((long long)L.nx_ * L.ny_));
-    out[t] = SV<S>::ld(A[L.at(V, v, i, j, k)]);
+    out[t] = SV<S>::ld(A[V == 13 ? L.atc(v, i, j, k) : L.at(V, v, i, j, k)]);
 }
 
 template <class S>

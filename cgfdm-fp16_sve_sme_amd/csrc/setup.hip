@@ -118,9 +118,9 @@ __global__ void k_medium(SetupParams q, MediumSrc ms, S* __restrict__ cjm) {
     medium_sample(q, ms, i, j, k, vs, vp, rho);
     double mu, lambda, buoy;
     medium_at(q, vs, vp, rho, mu, lambda, buoy);
-    cjm[q.lay.at(13, 10, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)mu);
-    cjm[q.lay.at(13, 11, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)lambda);
-    cjm[q.lay.at(13, 12, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)buoy);
+    cjm[q.lay.atc(10, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)mu);
+    cjm[q.lay.atc(11, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)lambda);
+    cjm[q.lay.atc(12, i + HALO, j + HALO, k + HALO)] = SV<S>::st((float)buoy);
 }
 
 // Largest Vp of the medium (for the CFL check), as an atomic maximum on the float bits.
@@ -151,8 +151,8 @@ __global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
     int kk = k < _nz ? k : 2 * (_nz - 1) - k;
     double m[9], J;
     metric_at(q, i, j, kk, m, J);
-    for (int c = 0; c < 9; ++c) cjm[q.lay.at(13, c, i, j, k)] = SV<S>::st((float)m[c]);
-    cjm[q.lay.at(13, 9, i, j, k)] = SV<S>::st((float)J);
+    for (int c = 0; c < 9; ++c) cjm[q.lay.atc(c, i, j, k)] = SV<S>::st((float)m[c]);
+    cjm[q.lay.atc(9, i, j, k)] = SV<S>::st((float)J);
 }
 
 // Free-surface matrices -rDZ^-1 DX and -rDZ^-1 DY at the surface layer

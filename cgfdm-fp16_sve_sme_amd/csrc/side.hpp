@@ -102,7 +102,7 @@ template <class S>
 __device__ __forceinline__ float source_coef_at(const S* __restrict__ cjm, const Layout& L, int SX, int SY, int SZ,
                                                 float amp, float DH, double CsDT) {
 #pragma clang fp contract(off)
-    const float jacb = SV<S>::ld(cjm[L.at(13, 9, SX, SY, SZ)]);
+    const float jacb = SV<S>::ld(cjm[L.atc(9, SX, SY, SZ)]);
     const float value = -1.0f * amp / (jacb * (DH * DH * DH));
     return (float)(value * CsDT);
 }

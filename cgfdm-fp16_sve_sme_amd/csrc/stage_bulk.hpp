@@ -1,5 +1,6 @@
 // stage_bulk.hpp — the hot kernel: one fused RK stage over the bulk of the grid (every point that
-// is neither in a PML slab nor in the top three free-surface layers).
+// is neither in a PML slab nor in the top three free-surface layers), and the same tile code for
+// the single-slab PML boxes.
 //
 // Work decomposition (CDNA4, wave64):
 //  - a workgroup owns a TX x TY column tile in (x, y) and marches a chunk of planes along z in the
@@ -7,20 +8,20 @@
 //    register queue and each plane of W is read from HBM once per chunk;
 //  - the current plane of the 9 fields, with its (4-wide) x/y halo, is staged in LDS as FP32 and
 //    double-buffered, so one barrier per plane suffices; x and y neighbours are LDS reads;
-//  - every global load of an iteration (the plane k+4FB3 entering the queue, the halo cells, the
-//    12 metric/medium values and the RK registers) is unconditional and issued at the top of the
-//    iteration, before any wait: no load sits in a branch, where the compiler would have to wait
-//    for it inside the branch and serialise the iteration's memory latency;
-//  - global accesses are buffer loads/stores: one descriptor per array and plane (block-
-//    interleaved [k][j][x block][var][64] layout, so the variables of a wave's 64 points are one
-//    contiguous run), a per-lane byte offset that does not change along the march and a constant
-//    per-variable offset, so no VALU address arithmetic is spent;
-//  - the RK stage is a template parameter, so the register update has no run-time branches;
-//  - the z loop is unrolled by 5 (the queue rotates by renaming) and always runs a multiple of 5
-//    iterations: the ones past the chunk load clamped planes and store nothing, so there is a
-//    single copy of the unrolled body.
+//  - global accesses are buffer loads/stores of variable pairs (the layout stores the two
+//    variables of a pair next to each other per point): one 32-bit (16-bit storage) or 64-bit
+//    (FP32 storage) access per lane brings two variables, with one descriptor per array and plane,
+//    a per-lane byte offset that does not change along the march and constant per-pair offsets;
+//  - 16-bit storage keeps the pairs packed in registers (two values per VGPR, widened to FP32 by
+//    the mixed-precision FMAs that consume them) and uses the registers this frees to request the
+//    next plane's halo cells and metric/medium values one iteration ahead, so the barrier of an
+//    iteration never waits on a load issued in the same iteration;
+//  - no load sits in a branch (the compiler would wait for it inside the branch); the RK stage is
+//    a template parameter; the z loop is unrolled by 5 (the queue rotates by renaming) and always
+//    runs a multiple of 5 iterations, the ones past the chunk loading clamped planes and storing
+//    nothing, so there is a single copy of the unrolled body.
 // Arithmetic is FP32 with the stencil coefficients pre-multiplied by FB*rDH and DT folded into
-// lambda, mu and the buoyancy. In the bulk every PML factor is 1, so it is not applied.
+// lambda, mu and the buoyancy.
 #pragma once
 #include <type_traits>
 
@@ -32,6 +33,84 @@ template <int TX, int TY> struct BulkLds {
     static constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and var stride
     float v[2][9 * VS];
 };
+
+// A pair of stored values (two variables of one point) as loaded by one lane, and a single one.
+template <class S> struct Pair;
+template <> struct Pair<_Float16> {
+    typedef _Float16 type __attribute__((ext_vector_type(2)));
+    typedef _Float16 one;
+    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(type, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+    }
+    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(one, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0));
+    }
+    __device__ __forceinline__ static float f1(one x) { return (float)x; }
+    __device__ __forceinline__ static float lo(type x) { return (float)x.x; }
+    __device__ __forceinline__ static float hi(type x) { return (float)x.y; }
+    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+        type x;
+        x.x = (_Float16)a;
+        x.y = (_Float16)b;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)voff, soff, 0);
+    }
+};
+template <> struct Pair<__bf16> {
+    typedef unsigned type;
+    typedef unsigned short one;
+    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0);
+    }
+    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0);
+    }
+    __device__ __forceinline__ static float f1(one x) { return __builtin_bit_cast(float, (unsigned)x << 16); }
+    __device__ __forceinline__ static float lo(type x) { return __builtin_bit_cast(float, x << 16); }
+    __device__ __forceinline__ static float hi(type x) { return __builtin_bit_cast(float, x & 0xffff0000u); }
+    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+        const unsigned ua = __builtin_bit_cast(unsigned short, (__bf16)a);
+        const unsigned ub = __builtin_bit_cast(unsigned short, (__bf16)b);
+        __builtin_amdgcn_raw_buffer_store_b32(ua | (ub << 16), r, (int)voff, soff, 0);
+    }
+};
+template <> struct Pair<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+    typedef float one;
+    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(type, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0));
+    }
+    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) { return bld<float>(r, voff, soff); }
+    __device__ __forceinline__ static float f1(one x) { return x; }
+    __device__ __forceinline__ static float lo(type x) { return x.x; }
+    __device__ __forceinline__ static float hi(type x) { return x.y; }
+    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+        type x;
+        x.x = a;
+        x.y = b;
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, soff, 0);
+    }
+};
+
+// The 9 stored wavefield values of one point: four pairs and the unpaired last variable.
+template <class S> struct W9 {
+    typename Pair<S>::type p[4];
+    typename Pair<S>::one s;
+    __device__ __forceinline__ float operator[](int v) const {
+        return v == 8 ? Pair<S>::f1(s) : (v & 1) ? Pair<S>::hi(p[v >> 1]) : Pair<S>::lo(p[v >> 1]);
+    }
+    // PB: bytes from one pair to the next; col/col1: byte offsets of pair 0 and of variable 8
+    template <int PB> __device__ __forceinline__ void load(Rsrc r, unsigned col, unsigned col1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[c] = Pair<S>::ld(r, col, c * PB);
+        s = Pair<S>::ld1(r, col1, 0);
+    }
+};
+template <class S, int PB> __device__ __forceinline__ void store9(Rsrc r, unsigned col, unsigned col1, const float* o) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Pair<S>::st(r, col, c * PB, o[2 * c], o[2 * c + 1]);
+    bst<S>(r, col1, 0, o[8]);
+}
 
 // One tile: columns [ci0 + bx*TX, +TX) x [cj0 + by*TY, +TY) of the box with columns
 // [ci0,ci1) x [cj0,cj1), planes [kb, ke). Every thread of the workgroup calls it with the same
@@ -47,8 +126,12 @@ template <class S, int TX, int TY, int IRK, int AXIS = -1>
 __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int ci0, int ci1, int cj0,
                                           int cj1, int sid, int bx, int by, int kb, int ke,
                                           float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+    using PR = Pair<S>;
+    using P2 = typename PR::type;
+    constexpr bool PREFETCH = sizeof(S) == 2;  // packed pairs leave room for the next plane
     constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
     constexpr int NT = TX * TY;
+    constexpr int PB = 2 * kBlk * (int)sizeof(S);  // bytes from one variable pair to the next
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -56,14 +139,13 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const int i = i0 + tx, j = j0 + ty;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
     const Layout L = p.lay;
-    const long long plane = L.plane;
     const bool has = i < L.nx_ && j < L.ny_;   // a grid point (possibly outside the box)
     const bool inside = i < ci1 && j < cj1;    // computed by this kernel
-    // byte offsets of the point in a k-plane of a 9-variable array (W, A, T, Wout) and of the
-    // 13-component metric/medium array; a variable adds vs bytes
+    // byte offsets of the point in a k-plane of a 9-variable array (W, A, T, Wout: first pair and
+    // variable 8) and of the metric/medium array; variable pair q adds q*PB bytes
     const unsigned col = has ? (unsigned)L.col(9, i, j) * (unsigned)sizeof(S) : 0u;
+    const unsigned col1 = has ? (unsigned)L.col1(9, i, j) * (unsigned)sizeof(S) : 0u;
     const unsigned colc = has ? (unsigned)L.col(13, i, j) * (unsigned)sizeof(S) : 0u;
-    constexpr int vs = kBlk * (int)sizeof(S);
 
     // z chunk, traversed in the direction FB3
     const int nk = ke - kb;
@@ -75,9 +157,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const char* __restrict__ Ab = (const char*)p.A;
     char* __restrict__ Tb = (char*)p.T;
     char* __restrict__ Ob = (char*)p.Wout;
-    const long long wpl = 9 * plane * (long long)sizeof(S);   // bytes per plane of W, A, T, Wout
-    const long long cpl = 13 * plane * (long long)sizeof(S);  // bytes per plane of the metric/medium
-    const long long wrange = 9 * plane, crange = 13 * plane;  // elements per plane
+    const long long wrange = L.kplane(9), crange = L.kplane(13);  // elements per k-plane
+    const long long wpl = wrange * (long long)sizeof(S), cpl = crange * (long long)sizeof(S);
 
     // stencil weights with direction and 1/DH folded in
     const float rDH = p.rDH;
@@ -89,26 +170,24 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 cz4 = FB3 * af3 * rDH;
 
     // register queue: slot (m + R) % 5 holds W(k + FB3*(m-1)), m = 0..4, at unrolled step R
-    float q[5][9];
+    W9<S> q[5];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        const Rsrc rm = make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange);
-#pragma unroll
-        for (int v = 0; v < 9; ++v) q[m][v] = bld<S>(rm, col, v * vs);
-    }
+    for (int m = 0; m < 5; ++m)
+        q[m].template load<PB>(make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange), col, col1);
 
     // halo cells of this thread: cell c = tid + r*NT (r < HPT) of the list x-halo (4 columns x TY
     // rows) then y-halo (4 rows x LX columns); a thread without a cell loads element 0 in vain
     constexpr int NHX = 4 * TY, NHY = 4 * LX, NH = NHX + NHY;
     constexpr int HPT = (NH + NT - 1) / NT;
     int hl[HPT];
-    unsigned hcol[HPT];
+    unsigned hcol[HPT], hcol1[HPT];
     bool hown[HPT];
 #pragma unroll
     for (int r = 0; r < HPT; ++r) {
         const int c = tid + r * NT;
         hl[r] = 0;
         hcol[r] = 0;
+        hcol1[r] = 0;
         hown[r] = c < NH;
         if (c < NH) {
             int hr, hc;
@@ -125,6 +204,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             const int ig = i0 - hx + hc, jg = j0 - hy + hr;
             const bool hhas = ig >= 0 && ig < L.nx_ && jg >= 0 && jg < L.ny_;
             hcol[r] = hhas ? (unsigned)L.col(9, ig, jg) * (unsigned)sizeof(S) : 0u;
+            hcol1[r] = hhas ? (unsigned)L.col1(9, ig, jg) * (unsigned)sizeof(S) : 0u;
             hl[r] = hr * LX + hc;
         }
     }
@@ -133,6 +213,21 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const float Crho = p.Crho;
     int k = k0;
 
+    // halo cells and metric/medium pairs of a plane (the metric array's six pairs are
+    // xi_x..zt_z, mu, lambda, buoyancy)
+    W9<S> hv[HPT];
+    P2 M[6];
+    [[maybe_unused]] W9<S> hvN[HPT];
+    [[maybe_unused]] P2 MN[6];
+    auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[6]) {
+        const Rsrc rW = make_rsrc((const S*)(Wb + (long long)kk * wpl), wrange);
+#pragma unroll
+        for (int r = 0; r < HPT; ++r) h[r].template load<PB>(rW, hcol[r], hcol1[r]);
+        const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kk * cpl), crange);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) m[c] = PR::ld(rC, colc, c * PB);
+    };
+    if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
     // PML slab of the box: per-lane profile values on the x/y axes, aux positions
     // pos = base + kk*kstride with kk the plane index inside the slab's k range
     [[maybe_unused]] float pBeta = 1.f, pD = 0.f, pAD = 0.f;
@@ -175,48 +270,41 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 
     auto step = [&](auto Rc, int it) {
         constexpr int R = decltype(Rc)::value;
-        float(&q0)[9] = q[(0 + R) % 5];
-        float(&q1)[9] = q[(1 + R) % 5];
-        float(&q2)[9] = q[(2 + R) % 5];
-        float(&q3)[9] = q[(3 + R) % 5];
-        float(&q4)[9] = q[(4 + R) % 5];
+        W9<S>& q0 = q[(0 + R) % 5];
+        const W9<S>& q1 = q[(1 + R) % 5];
+        const W9<S>& q2 = q[(2 + R) % 5];
+        const W9<S>& q3 = q[(3 + R) % 5];
+        const W9<S>& q4 = q[(4 + R) % 5];
         const bool active = it < nk;  // iterations past the chunk store nothing
         float* Lb = lds[it & 1];
         const int kc = min(max(k, 0), kmax);  // clamped plane (only the inactive iterations need it)
         const long long pw = (long long)kc * wpl;
-        const Rsrc rW = make_rsrc((const S*)(Wb + pw), wrange);
-        float hv[HPT][9];
+        if constexpr (!PREFETCH) fetch_plane(kc, hv, M);
+        // stage the centre plane and its halo in LDS
+        float cen[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            cen[v] = q1[v];
+            Lb[v * VS + lc] = cen[v];
+        }
 #pragma unroll
         for (int r = 0; r < HPT; ++r)
+            if (hown[r]) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hv[r][v] = bld<S>(rW, hcol[r], v * vs);
-        // the plane entering the queue (clamped at the grid end; unused there)
-        float nq[9];
-        {
-            const int kn = min(max(k + 4 * FB3, 0), kmax);
-            const Rsrc rN = make_rsrc((const S*)(Wb + (long long)kn * wpl), wrange);
-#pragma unroll
-            for (int v = 0; v < 9; ++v) nq[v] = bld<S>(rN, col, v * vs);
-        }
-        float M[12];
-        {
-            const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kc * cpl), crange);
-#pragma unroll
-            for (int c = 0; c < 9; ++c) M[c] = bld<S>(rC, colc, c * vs);
-#pragma unroll
-            for (int c = 10; c < 13; ++c) M[c - 1] = bld<S>(rC, colc, c * vs);
-        }
+                for (int v = 0; v < 9; ++v) Lb[v * VS + hl[r]] = hv[r][v];
+            }
+        __syncthreads();
+        // requests for later iterations: the plane entering the queue and, with prefetch, the
+        // next plane's halo and metric/medium pairs (clamped at the grid end; unused there)
+        W9<S> nq;
+        nq.template load<PB>(make_rsrc((const S*)(Wb + (long long)min(max(k + 4 * FB3, 0), kmax) * wpl), wrange), col,
+                             col1);
+        if constexpr (PREFETCH) fetch_plane(min(max(k + FB3, 0), kmax), hvN, MN);
+        // RK registers of this plane: needed only by the final update
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
-        float ra[9], rt[9];
-        if constexpr (IRK == 1 || IRK == 2) {
-            const Rsrc rA = make_rsrc((const S*)(Ab + pw), wrange);
-#pragma unroll
-            for (int v = 0; v < 9; ++v) ra[v] = bld<S>(rA, col, v * vs);
-        }
-        if constexpr (IRK >= 1) {
-#pragma unroll
-            for (int v = 0; v < 9; ++v) rt[v] = bld<S>(rT, col, v * vs);
-        }
+        W9<S> ra, rt;
+        if constexpr (IRK == 1 || IRK == 2) ra.template load<PB>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
+        if constexpr (IRK >= 1) rt.template load<PB>(rT, col, col1);
         // auxiliaries of this plane (lanes outside the box read element 0)
         [[maybe_unused]] float xa[9], xm[9], xt[9];
         [[maybe_unused]] unsigned xoff = 0;
@@ -243,18 +331,11 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 zAD = zD + p.pa[2][kc];
             }
         }
-        // stage the centre plane and its halo in LDS
-#pragma unroll
-        for (int v = 0; v < 9; ++v) Lb[v * VS + lc] = q1[v];
-#pragma unroll
-        for (int r = 0; r < HPT; ++r)
-            if (hown[r]) {
-#pragma unroll
-                for (int v = 0; v < 9; ++v) Lb[v * VS + hl[r]] = hv[r][v];
-            }
-        __syncthreads();
         if (inside && active) {
-            const float mu = M[9] * DT, lam = M[10] * DT, bu = M[11] * (Crho * DT);
+            float Mf[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Mf[c] = (c & 1) ? PR::hi(M[c >> 1]) : PR::lo(M[c >> 1]);
+            const float mu = PR::hi(M[4]) * DT, lam = PR::lo(M[5]) * DT, bu = PR::hi(M[5]) * (Crho * DT);
             const int ox0 = lc - FB1, ox2 = lc + FB1, ox3 = lc + 2 * FB1, ox4 = lc + 3 * FB1;
             const int oy0 = lc - FB2 * LX, oy2 = lc + FB2 * LX, oy3 = lc + 2 * FB2 * LX, oy4 = lc + 3 * FB2 * LX;
             // The 27 one-sided derivatives are folded into the physical velocity gradient
@@ -266,7 +347,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const float* Lv = Lb + v * VS;
-                const float c = q1[v];
+                const float c = cen[v];
                 float d0 = cx0 * Lv[ox0] + cx1 * c + cx2 * Lv[ox2] + cx3 * Lv[ox3] + cx4 * Lv[ox4];
                 float d1 = cy0 * Lv[oy0] + cy1 * c + cy2 * Lv[oy2] + cy3 * Lv[oy3] + cy4 * Lv[oy4];
                 float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
@@ -275,14 +356,14 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 if constexpr (AXIS == 2) { draw[v] = d2; d2 *= zBeta; }
                 if (v < 3) {
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) G[v][cc] = M[cc] * d0 + M[3 + cc] * d1 + M[6 + cc] * d2;
+                    for (int cc = 0; cc < 3; ++cc) G[v][cc] = Mf[cc] * d0 + Mf[3 + cc] * d1 + Mf[6 + cc] * d2;
                 } else {
                     // stress component (row r0, column c0) contributes d/dx_c0 to dv[r0] and, off the
                     // diagonal, d/dx_r0 to dv[c0]
                     const int r0 = v == 3 ? 0 : v == 4 ? 1 : v == 5 ? 2 : v == 6 ? 0 : v == 7 ? 0 : 1;
                     const int c0 = v == 3 ? 0 : v == 4 ? 1 : v == 5 ? 2 : v == 6 ? 1 : v == 7 ? 2 : 2;
-                    dv[r0] += M[c0] * d0 + M[3 + c0] * d1 + M[6 + c0] * d2;
-                    if (v >= 6) dv[c0] += M[r0] * d0 + M[3 + r0] * d1 + M[6 + r0] * d2;
+                    dv[r0] += Mf[c0] * d0 + Mf[3 + c0] * d1 + Mf[6 + c0] * d2;
+                    if (v >= 6) dv[c0] += Mf[r0] * d0 + Mf[3 + r0] * d1 + Mf[6 + r0] * d2;
                 }
             }
             float h[9];
@@ -305,7 +386,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 // (with d, no beta) of the slab normal
 #pragma unroll
                 for (int v = 0; v < 9; ++v) h[v] += -beta * xa[v] * DT;
-                const float mx = M[3 * AXIS], my = M[3 * AXIS + 1], mz = M[3 * AXIS + 2];
+                const float mx = Mf[3 * AXIS], my = Mf[3 * AXIS + 1], mz = Mf[3 * AXIS + 2];
                 float b[9];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) b[v] = draw[v] * dd;
@@ -340,26 +421,33 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             }
             // RK registers (wave_rk0..3)
             const Rsrc rO = make_rsrc((const S*)(Ob + pw), wrange);
+            float o[9], t[9];
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 if constexpr (IRK == 0) {
-                    const float m0 = q1[v];
-                    bst<S>(rO, col, v * vs, m0 + alpha2 * h[v]);
-                    bst<S>(rT, col, v * vs, m0 + beta1 * h[v]);
+                    o[v] = cen[v] + alpha2 * h[v];
+                    t[v] = cen[v] + beta1 * h[v];
                 } else if constexpr (IRK == 1) {
-                    bst<S>(rO, col, v * vs, ra[v] + alpha3 * h[v]);
-                    bst<S>(rT, col, v * vs, rt[v] + beta2 * h[v]);
+                    o[v] = ra[v] + alpha3 * h[v];
+                    t[v] = rt[v] + beta2 * h[v];
                 } else if constexpr (IRK == 2) {
-                    bst<S>(rO, col, v * vs, ra[v] + h[v]);
-                    bst<S>(rT, col, v * vs, rt[v] + beta3 * h[v]);
+                    o[v] = ra[v] + h[v];
+                    t[v] = rt[v] + beta3 * h[v];
                 } else {
-                    bst<S>(rO, col, v * vs, rt[v] + beta4 * h[v]);
+                    o[v] = rt[v] + beta4 * h[v];
                 }
             }
+            store9<S, PB>(rO, col, col1, o);
+            if constexpr (IRK <= 2) store9<S, PB>(rT, col, col1, t);
         }
         // the slot of the oldest plane receives the new one
+        q0 = nq;
+        if constexpr (PREFETCH) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) q0[v] = nq[v];
+            for (int r = 0; r < HPT; ++r) hv[r] = hvN[r];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) M[c] = MN[c];
+        }
         k += FB3;
     };
     using I0 = std::integral_constant<int, 0>;

@@ -35,10 +35,9 @@ __device__ __forceinline__ void rk_update(const StageParams& p, int i, int j, in
     S* __restrict__ Wo = (S*)p.Wout;
     const S* __restrict__ A = (const S*)p.A;
     S* __restrict__ T = (S*)p.T;
-    const long long q0 = p.lay.at(9, 0, i, j, k);
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-        const long long q = q0 + v * kBlk;
+        const long long q = p.lay.at(9, v, i, j, k);
         if (p.irk == 0) {
             float m = SV<S>::ld(A[q]);
             Wo[q] = SV<S>::st(m + alpha2 * h[v]);
@@ -64,7 +63,7 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
     const S* __restrict__ Cj = (const S*)p.cjm;
     const Layout& L = p.lay;
     auto w = [&](int v, int ii, int jj, int kk) { return SV<S>::ld(W[L.at(9, v, ii, jj, kk)]); };
-    auto cj = [&](int c, int ii, int jj, int kk) { return SV<S>::ld(Cj[L.at(13, c, ii, jj, kk)]); };
+    auto cj = [&](int c, int ii, int jj, int kk) { return SV<S>::ld(Cj[L.atc(c, ii, jj, kk)]); };
     const long long rdzi = (i + (long long)j * p.nx_) * 9;
     const float rDH = p.rDH, DT = p.DT;
     const int _nz = p.nz + HALO;
