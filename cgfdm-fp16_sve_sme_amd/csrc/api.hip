@@ -845,6 +845,10 @@ int cgfdm_decompose(const cgfdm_desc* desc, int* boxes, int max_boxes) {
         put(0, -1, B);
     }
     for (int b = 0; b < q.npbox; ++b) put(1, q.pbox[b][7], q.pbox[b]);
+    if (q.fs_on) {
+        const int B[6] = {q.bi0, q.bi1, q.bj0, q.bj1, q.fk0, q.fk1};
+        put(3, -1, B);
+    }
     for (int b = 0; b < q.nbox; ++b) put(2, -1, q.box[b]);
     return n;
 }
@@ -854,7 +858,7 @@ int cgfdm_partition(const cgfdm_domain* h, long long* out) {
     StageParams q = domain_stage_params(h);
     stage_boxes(q, 1);
     out[0] = q.bulk_on ? (long long)(q.bi1 - q.bi0) * (q.bj1 - q.bj0) * (q.bk1 - q.bk0) : 0;
-    out[1] = q.boxStart[q.nbox];
+    out[1] = q.boxStart[q.nbox] + (q.fs_on ? (long long)(q.bi1 - q.bi0) * (q.bj1 - q.bj0) * (q.fk1 - q.fk0) : 0);
     out[2] = h->bulk_cfg;
     out[3] = h->bulk_kc;
     return CGFDM_OK;

@@ -9,8 +9,8 @@
 // Each domain keeps its own step counter on the device; k_advance moves it and the MacCormack
 // signs of the next step, and a domain whose counter reached its NT is skipped by every kernel.
 //
-// The per-point arithmetic is the single-domain code (bulk_tile, stage_point_general and the
-// side bodies), so a domain computes bit-identical results alone and in any mix.
+// The per-point arithmetic is the single-domain code (bulk_tile, fs_tile, stage_point_general
+// and the side bodies), so a domain computes bit-identical results alone and in any mix.
 #include "batch.hpp"
 #include "side.hpp"
 #include "stage_general.hpp"
@@ -68,7 +68,7 @@ __global__ void k_clear_slot(BatchDev b, int s) {
 // Prefix sums of bulk tiles and general chunks over the active slots, in slot order.
 __global__ void k_build_tables(BatchDev b) {
     if (threadIdx.x != 0) return;
-    int nb = 0, ng = 0, tiles = 0;
+    int nb = 0, ng = 0, tiles = 0, nf = 0, ftiles = 0;
     int np[3] = {0, 0, 0}, ptiles[3] = {0, 0, 0};
     long long chunks = 0;
     for (int s = 0; s < b.maxSlots; ++s) {
@@ -78,6 +78,11 @@ __global__ void k_build_tables(BatchDev b) {
             BulkEnt& e = b.tab->bulk[nb++];
             e.slot = s; e.gx = sd.gx; e.gy = sd.gy; e.nch = sd.nch; e.kc = sd.kc; e.start = tiles;
             tiles += sd.gx * sd.gy * sd.nch;
+        }
+        if (sd.fgx > 0) {
+            FsEnt& e = b.tab->fs[nf++];
+            e.slot = s; e.gx = sd.fgx; e.gy = sd.fgy; e.start = ftiles;
+            ftiles += sd.fgx * sd.fgy;
         }
         for (int x = 0; x < sd.nbox; ++x) {
             GenEnt& e = b.tab->gen[ng++];
@@ -98,6 +103,8 @@ __global__ void k_build_tables(BatchDev b) {
     b.tab->nTiles = tiles;
     b.tab->nGen = ng;
     b.tab->nChunks = chunks;
+    b.tab->nFs = nf;
+    b.tab->nFsTiles = ftiles;
     for (int a = 0; a < 3; ++a) {
         b.tab->nPml[a] = np[a];
         b.tab->nPmlTiles[a] = ptiles[a];
@@ -167,6 +174,22 @@ __global__ void __launch_bounds__(TX* TY) k_pml_multi(BatchDev b) {
     }
 }
 
+template <class S, int IRK>
+__global__ void __launch_bounds__(kFsTx* kFsTy) k_fs_multi(BatchDev b) {
+    __shared__ FsLds<kFsTx, kFsTy> lds;
+    const int nEnt = b.tab->nFs, nTiles = b.tab->nFsTiles;
+    for (int w = blockIdx.x; w < nTiles; w += gridDim.x) {
+        const int t = (int)xcd_order(w, nTiles);
+        const FsEnt& e = b.tab->fs[find_entry(b.tab->fs, nEnt, t)];
+        const SlotDev& sd = b.slots[e.slot];
+        if (sd.it >= sd.NT) continue;
+        const StageParams& p = b.sp[e.slot * 4 + IRK];
+        const int r = t - e.start;
+        fs_tile<S, kFsTx, kFsTy>(p, r % e.gx, r / e.gx, lds);
+        __syncthreads();  // the next tile rewrites the LDS plane
+    }
+}
+
 template <class S>
 __global__ void __launch_bounds__(kGenChunk) k_general_multi(BatchDev b, int irk) {
     const int nEnt = b.tab->nGen;
@@ -232,6 +255,7 @@ static void batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
     k_pml_multi<S, 16, 16, IRK, 0><<<g.pmlBlocks[0], 256, 0, st>>>(b);
     k_pml_multi<S, 64, 4, IRK, 1><<<g.pmlBlocks[1], 256, 0, st>>>(b);
     k_pml_multi<S, 64, 4, IRK, 2><<<g.pmlBlocks[2], 256, 0, st>>>(b);
+    k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, st>>>(b);
     k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, IRK);
 }
 
@@ -261,7 +285,7 @@ int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStre
 
 // Persistent grid sizes: as many workgroups as fit on the device at once.
 int batch_grid(int prec, int cfg, BatchGrid& g) {
-    int dev = 0, ncu = 0, nb = 0, ng = 0, n0 = 0, n1 = 0, n2 = 0;
+    int dev = 0, ncu = 0, nb = 0, ng = 0, n0 = 0, n1 = 0, n2 = 0, nf = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -3;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -3;
     hipError_t e1 = hipSuccess, e2 = hipSuccess;
@@ -272,7 +296,8 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);                  \
     if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_pml_multi<S, 16, 16, 1, 0>, 256, 0); \
     if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_pml_multi<S, 64, 4, 1, 1>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, k_pml_multi<S, 64, 4, 1, 2>, 256, 0);
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, k_pml_multi<S, 64, 4, 1, 2>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nf, k_fs_multi<S, 1>, kFsTx * kFsTy, 0);
     switch (prec) {
         case P_F32: CG_OCC(float) break;
         case P_F16: CG_OCC(_Float16) break;
@@ -286,6 +311,7 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     g.genBlocks = (ncu * (ng > 0 ? ng : 1) + kXcds - 1) / kXcds * kXcds;
     const int npm[3] = {n0, n1, n2};
     for (int a = 0; a < 3; ++a) g.pmlBlocks[a] = (ncu * (npm[a] > 0 ? npm[a] : 1) + kXcds - 1) / kXcds * kXcds;
+    g.fsBlocks = (ncu * (nf > 0 ? nf : 1) + kXcds - 1) / kXcds * kXcds;
     return 0;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include "common.hpp"
 #include "stage_bulk.hpp"
+#include "stage_fs.hpp"
 
 namespace cg {
 
@@ -15,6 +16,7 @@ struct SlotDev {
     int fs;
     // bulk tiling: gx x gy column tiles, nch chunks of kc planes; general: chunks per box
     int gx, gy, nch, kc;
+    int fgx, fgy;       // free-surface tiles over the bulk columns (0: no free surface)
     PmlTiling pt[3];    // single-slab PML boxes by normal axis
     int nbox;
     long long boxChunks[kMaxGenBoxes];
@@ -43,6 +45,7 @@ struct SlotDev {
 struct BulkEnt { int slot, gx, gy, nch, kc, start; };
 struct PmlEnt { int slot, box, gx, gy, nch, kc, start; };
 struct GenEnt { int slot, box; long long start; };
+struct FsEnt { int slot, gx, gy, start; };
 
 struct BatchTables {
     int nBulk, nTiles, nGen, pad;
@@ -51,6 +54,8 @@ struct BatchTables {
     BulkEnt bulk[kMaxSlots];
     PmlEnt pml[3][kMaxSlots * 2];
     GenEnt gen[kMaxSlots * kMaxGenBoxes];
+    int nFs, nFsTiles;
+    FsEnt fs[kMaxSlots];
 };
 
 // Device pointers of one batch (passed by value to every batch kernel).
@@ -61,7 +66,7 @@ struct BatchDev {
     int maxSlots;
 };
 
-struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[3]; };
+struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[3], fsBlocks; };
 
 void stage_boxes(StageParams& q, int use_fast);
 PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY);

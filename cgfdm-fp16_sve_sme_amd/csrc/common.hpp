@@ -160,11 +160,13 @@ struct StageParams {
     //    by the tile kernel;
     //  - npbox single-slab PML boxes (the part of a slab outside every other slab and the
     //    free-surface layers), covered by the tile kernel with that slab's auxiliary update;
+    //  - the free-surface box [bi0,bi1)x[bj0,bj1)x[fk0,fk1): the three free-surface layers above
+    //    the bulk columns (no PML), covered by the free-surface tile kernel;
     //  - nbox general boxes, covered point by point: the edges and corners where slabs meet and
-    //    the free-surface layers.
-    int bulk_on;
+    //    the free-surface layers of the PML columns.
+    int bulk_on, fs_on;
     int bulk_cfg, bulk_kc;  // tile shape index and z-chunk length (0: automatic)
-    int bi0, bi1, bj0, bj1, bk0, bk1;
+    int bi0, bi1, bj0, bj1, bk0, bk1, fk0, fk1;
     int npbox;
     int pbox[6][8];         // i0,i1,j0,j1,k0,k1, slab index (0..5), axis (0..2)
     int nbox;

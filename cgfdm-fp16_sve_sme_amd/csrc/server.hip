@@ -248,6 +248,10 @@ int cgfdm_server::setup_resident(Resident& R) {
         sd.kc = std::min(nzb, 16);
         sd.nch = (nzb + sd.kc - 1) / sd.kc;
     }
+    if (base.fs_on) {
+        sd.fgx = (base.bi1 - base.bi0 + kFsTx - 1) / kFsTx;
+        sd.fgy = (base.bj1 - base.bj0 + kFsTy - 1) / kFsTy;
+    }
     for (int a = 0; a < 3; ++a) sd.pt[a] = pml_tiling(base, a, a == 0 ? 16 : 64, a == 0 ? 16 : 4);
     sd.nbox = base.nbox;
     for (int x = 0; x < base.nbox; ++x)

@@ -2,6 +2,7 @@
 // the general kernel over the PML slabs and free-surface layers around it.
 #include "stage_general.hpp"
 #include "stage_bulk.hpp"
+#include "stage_fs.hpp"
 
 namespace cg {
 
@@ -64,6 +65,7 @@ void stage_boxes(StageParams& q, int use_fast) {
     const int K2 = q.fs ? hiK - HALO : hiK - np;  // start of the top range
     q.nbox = 0;
     q.npbox = 0;
+    q.fs_on = 0;
     q.boxStart[0] = 0;
     q.bi0 = I1; q.bi1 = I2;
     q.bj0 = J1; q.bj1 = J2;
@@ -93,7 +95,15 @@ void stage_boxes(StageParams& q, int use_fast) {
     add_box(q, I2, hiI, J0, J1, K1, K2);
     add_box(q, I2, hiI, J2, hiJ, K1, K2);
     if (q.fs) {
-        add_box(q, lo, hiI, lo, hiJ, K2, hiK);  // free-surface layers
+        // free-surface layers: above the bulk columns by the free-surface tile kernel, the ring
+        // of PML columns by the general kernel
+        q.fs_on = 1;
+        q.fk0 = K2;
+        q.fk1 = hiK;
+        add_box(q, lo, hiI, lo, J1, K2, hiK);
+        add_box(q, lo, hiI, J2, hiJ, K2, hiK);
+        add_box(q, lo, I1, J1, J2, K2, hiK);
+        add_box(q, I2, hiI, J1, J2, K2, hiK);
     } else {
         add_box(q, lo, hiI, J0, J1, K2, hiK);
         add_box(q, lo, hiI, J2, hiJ, K2, hiK);
@@ -112,7 +122,7 @@ PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY) {
         if (B[7] != axis) continue;
         const int e = t.n++;
         t.box[e] = b;
-        t.gx[e] = (B[1] - B[0] + TX - 1) / TX;
+        t.gx[e] = (B[1] - pml_tile_x0(q.lay, B[0], TX) + TX - 1) / TX;
         t.gy[e] = (B[3] - B[2] + TY - 1) / TY;
         cols += t.gx[e] * t.gy[e];
     }
@@ -169,6 +179,10 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
         launch_pml<S, 64, 4, 1>(q, s);
         launch_pml<S, 64, 4, 2>(q, s);
         if (hipGetLastError() != hipSuccess) return -3;
+    }
+    if (q.fs_on && use_fast != 2) {
+        const int gx = (q.bi1 - q.bi0 + kFsTx - 1) / kFsTx, gy = (q.bj1 - q.bj0 + kFsTy - 1) / kFsTy;
+        k_stage_fs<S, kFsTx, kFsTy><<<(unsigned)(gx * gy), kFsTx * kFsTy, 0, s>>>(q, gx, gy);
     }
     const long long n = q.boxStart[q.nbox];
     if (n > 0 && use_fast != 2) {

@@ -29,6 +29,12 @@
 
 namespace cg {
 
+// First column of the x tiles of a box starting at column ci0: ci0 moved down to the start of
+// its TX-aligned run inside a layout block (TX divides the block size).
+__host__ __device__ __forceinline__ int pml_tile_x0(const Layout& L, int ci0, int TX) {
+    return ci0 - ((ci0 + L.lead) & (TX - 1));
+}
+
 template <int TX, int TY> struct BulkLds {
     static constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and var stride
     float v[2][9 * VS];
@@ -126,6 +132,7 @@ template <class S, int TX, int TY, int IRK, int AXIS = -1>
 __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int ci0, int ci1, int cj0,
                                           int cj1, int sid, int bx, int by, int kb, int ke,
                                           float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+    static_assert(kBlk % TX == 0, "a tile row must lie inside one layout block");
     using PR = Pair<S>;
     using P2 = typename PR::type;
     constexpr bool PREFETCH = sizeof(S) == 2;  // packed pairs leave room for the next plane
@@ -135,12 +142,14 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int i0 = ci0 + bx * TX, j0 = cj0 + by * TY;
+    const Layout L = p.lay;
+    // x tiles of a PML box start on a TX-aligned run of a layout block (pml_tile_x0), so a row of
+    // a tile never straddles two blocks; the bulk box starts a block
+    const int i0 = (AXIS >= 0 ? pml_tile_x0(L, ci0, TX) : ci0) + bx * TX, j0 = cj0 + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
-    const Layout L = p.lay;
-    const bool has = i < L.nx_ && j < L.ny_;   // a grid point (possibly outside the box)
-    const bool inside = i < ci1 && j < cj1;    // computed by this kernel
+    const bool has = i >= 0 && i < L.nx_ && j < L.ny_;       // a grid point (possibly outside the box)
+    const bool inside = i >= ci0 && i < ci1 && j < cj1;      // computed by this kernel
     // byte offsets of the point in a k-plane of a 9-variable array (W, A, T, Wout: first pair and
     // variable 8) and of the metric/medium array; variable pair q adds q*PB bytes
     const unsigned col = has ? (unsigned)L.col(9, i, j) * (unsigned)sizeof(S) : 0u;

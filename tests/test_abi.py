@@ -89,8 +89,9 @@ def test_no_fallback_when_library_missing(gpu, monkeypatch):
                                                (1, 0, 6, (20, 18, 12)), (0, 0, 5, (16, 16, 16)),
                                                (1, 1, 8, (18, 40, 30)), (1, 1, 4, (9, 9, 10))])
 def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
-    """Bulk box, single-slab PML boxes and general boxes tile the interior exactly once; a PML
-    box lies inside exactly one slab (its normal axis) and outside the free-surface layers."""
+    """Bulk box, single-slab PML boxes, the free-surface box and general boxes tile the interior
+    exactly once; a PML box lies inside exactly one slab (its normal axis) and outside the
+    free-surface layers; the free-surface box is the top three layers of the bulk columns."""
     import numpy as np
     NX, NY, NZ = shape
     d = gpu.make_desc(NX, NY, NZ, nPML=npml, free_surface=fs, pml=pml)
@@ -111,9 +112,14 @@ def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
                     assert b0 >= 3 + npml and b1 <= top
             if fs:
                 assert k1 <= 3 + NZ - 3
+        if kind == 3:  # free-surface layers above the bulk columns
+            assert fs and (k0, k1) == (3 + NZ - 3, 3 + NZ)
+            np_ = npml if pml else 0
+            assert (i0, i1, j0, j1) == (3 + np_, 3 + NX - np_, 3 + np_, 3 + NY - np_)
     inner = cover[3:NZ + 3, 3:NY + 3, 3:NX + 3]
     assert inner.min() == 1 and inner.max() == 1
     assert cover.sum() == NX * NY * NZ
     kinds = [b[6] for b in boxes]
     if pml and min(NX, NY, NZ) > 2 * npml + 3:
         assert kinds.count(0) == 1 and kinds.count(1) == (5 if fs else 6)
+        assert kinds.count(3) == (1 if fs else 0)
