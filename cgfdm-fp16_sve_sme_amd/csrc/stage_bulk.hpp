@@ -40,61 +40,96 @@ template <int TX, int TY> struct BulkLds {
     float v[2][9 * VS];
 };
 
+// Cache policy of a buffer access: kCpNt marks a stream the kernel touches once (non-temporal:
+// its lines are placed for early eviction from L2), kCpDefault one that neighbours re-read.
+constexpr int kCpDefault = 0, kCpNt = 2;
+
+// Raw buffer words with a compile-time cache policy.
+template <int CP> __device__ __forceinline__ unsigned short ldw16(Rsrc r, unsigned voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, CP);
+}
+template <int CP> __device__ __forceinline__ unsigned ldw32(Rsrc r, unsigned voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, CP);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <int CP> __device__ __forceinline__ u32x2 ldw64(Rsrc r, unsigned voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, CP);
+}
+template <int CP> __device__ __forceinline__ void stw16(Rsrc r, unsigned voff, int soff, unsigned short x) {
+    __builtin_amdgcn_raw_buffer_store_b16(x, r, (int)voff, soff, CP);
+}
+template <int CP> __device__ __forceinline__ void stw32(Rsrc r, unsigned voff, int soff, unsigned x) {
+    __builtin_amdgcn_raw_buffer_store_b32(x, r, (int)voff, soff, CP);
+}
+template <int CP> __device__ __forceinline__ void stw64(Rsrc r, unsigned voff, int soff, u32x2 x) {
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, (int)voff, soff, CP);
+}
+
 // A pair of stored values (two variables of one point) as loaded by one lane, and a single one.
 template <class S> struct Pair;
 template <> struct Pair<_Float16> {
     typedef _Float16 type __attribute__((ext_vector_type(2)));
     typedef _Float16 one;
-    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
-        return __builtin_bit_cast(type, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+    template <int CP> __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(type, ldw32<CP>(r, voff, soff));
     }
-    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
-        return __builtin_bit_cast(one, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0));
+    template <int CP> __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(one, ldw16<CP>(r, voff, soff));
     }
     __device__ __forceinline__ static float f1(one x) { return (float)x; }
     __device__ __forceinline__ static float lo(type x) { return (float)x.x; }
     __device__ __forceinline__ static float hi(type x) { return (float)x.y; }
-    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+    template <int CP> __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
         type x;
         x.x = (_Float16)a;
         x.y = (_Float16)b;
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)voff, soff, 0);
+        stw32<CP>(r, voff, soff, __builtin_bit_cast(unsigned, x));
+    }
+    template <int CP> __device__ __forceinline__ static void st1(Rsrc r, unsigned voff, int soff, float a) {
+        stw16<CP>(r, voff, soff, __builtin_bit_cast(unsigned short, (_Float16)a));
     }
 };
 template <> struct Pair<__bf16> {
     typedef unsigned type;
     typedef unsigned short one;
-    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0);
+    template <int CP> __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return ldw32<CP>(r, voff, soff);
     }
-    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
-        return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0);
+    template <int CP> __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
+        return ldw16<CP>(r, voff, soff);
     }
     __device__ __forceinline__ static float f1(one x) { return __builtin_bit_cast(float, (unsigned)x << 16); }
     __device__ __forceinline__ static float lo(type x) { return __builtin_bit_cast(float, x << 16); }
     __device__ __forceinline__ static float hi(type x) { return __builtin_bit_cast(float, x & 0xffff0000u); }
-    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+    template <int CP> __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
         const unsigned ua = __builtin_bit_cast(unsigned short, (__bf16)a);
         const unsigned ub = __builtin_bit_cast(unsigned short, (__bf16)b);
-        __builtin_amdgcn_raw_buffer_store_b32(ua | (ub << 16), r, (int)voff, soff, 0);
+        stw32<CP>(r, voff, soff, ua | (ub << 16));
+    }
+    template <int CP> __device__ __forceinline__ static void st1(Rsrc r, unsigned voff, int soff, float a) {
+        stw16<CP>(r, voff, soff, __builtin_bit_cast(unsigned short, (__bf16)a));
     }
 };
 template <> struct Pair<float> {
     typedef float type __attribute__((ext_vector_type(2)));
     typedef float one;
-    __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
-        return __builtin_bit_cast(type, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0));
+    template <int CP> __device__ __forceinline__ static type ld(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(type, ldw64<CP>(r, voff, soff));
     }
-    __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) { return bld<float>(r, voff, soff); }
+    template <int CP> __device__ __forceinline__ static one ld1(Rsrc r, unsigned voff, int soff) {
+        return __builtin_bit_cast(float, ldw32<CP>(r, voff, soff));
+    }
     __device__ __forceinline__ static float f1(one x) { return x; }
     __device__ __forceinline__ static float lo(type x) { return x.x; }
     __device__ __forceinline__ static float hi(type x) { return x.y; }
-    __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
+    template <int CP> __device__ __forceinline__ static void st(Rsrc r, unsigned voff, int soff, float a, float b) {
         type x;
         x.x = a;
         x.y = b;
-        typedef unsigned u2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, soff, 0);
+        stw64<CP>(r, voff, soff, __builtin_bit_cast(u32x2, x));
+    }
+    template <int CP> __device__ __forceinline__ static void st1(Rsrc r, unsigned voff, int soff, float a) {
+        stw32<CP>(r, voff, soff, __builtin_bit_cast(unsigned, a));
     }
 };
 
@@ -106,16 +141,17 @@ template <class S> struct W9 {
         return v == 8 ? Pair<S>::f1(s) : (v & 1) ? Pair<S>::hi(p[v >> 1]) : Pair<S>::lo(p[v >> 1]);
     }
     // PB: bytes from one pair to the next; col/col1: byte offsets of pair 0 and of variable 8
-    template <int PB> __device__ __forceinline__ void load(Rsrc r, unsigned col, unsigned col1) {
+    template <int PB, int CP = kCpDefault> __device__ __forceinline__ void load(Rsrc r, unsigned col, unsigned col1) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) p[c] = Pair<S>::ld(r, col, c * PB);
-        s = Pair<S>::ld1(r, col1, 0);
+        for (int c = 0; c < 4; ++c) p[c] = Pair<S>::template ld<CP>(r, col, c * PB);
+        s = Pair<S>::template ld1<CP>(r, col1, 0);
     }
 };
-template <class S, int PB> __device__ __forceinline__ void store9(Rsrc r, unsigned col, unsigned col1, const float* o) {
+template <class S, int PB, int CP>
+__device__ __forceinline__ void store9(Rsrc r, unsigned col, unsigned col1, const float* o) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) Pair<S>::st(r, col, c * PB, o[2 * c], o[2 * c + 1]);
-    bst<S>(r, col1, 0, o[8]);
+    for (int c = 0; c < 4; ++c) Pair<S>::template st<CP>(r, col, c * PB, o[2 * c], o[2 * c + 1]);
+    Pair<S>::template st1<CP>(r, col1, 0, o[8]);
 }
 
 // One tile: columns [ci0 + bx*TX, +TX) x [cj0 + by*TY, +TY) of the box with columns
@@ -139,6 +175,11 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
     constexpr int NT = TX * TY;
     constexpr int PB = 2 * kBlk * (int)sizeof(S);  // bytes from one variable pair to the next
+    // cache policies (measured): the metric/medium values, the RK registers A and T and the
+    // stage output are touched once per point by this kernel and go non-temporal (12 % less
+    // time than cached); the wavefield planes, whose halo cells neighbouring tiles re-read, and
+    // the PML auxiliaries stay cached (non-temporal made both slower)
+    constexpr int CPM = kCpNt, CPS = kCpNt, CPO = kCpNt, CPW = kCpDefault, CPH = kCpDefault, CPX = kCpDefault;
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -182,7 +223,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     W9<S> q[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m)
-        q[m].template load<PB>(make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange), col, col1);
+        q[m].template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange), col, col1);
 
     // halo cells of this thread: cell c = tid + r*NT (r < HPT) of the list x-halo (4 columns x TY
     // rows) then y-halo (4 rows x LX columns); a thread without a cell loads element 0 in vain
@@ -231,10 +272,10 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[6]) {
         const Rsrc rW = make_rsrc((const S*)(Wb + (long long)kk * wpl), wrange);
 #pragma unroll
-        for (int r = 0; r < HPT; ++r) h[r].template load<PB>(rW, hcol[r], hcol1[r]);
+        for (int r = 0; r < HPT; ++r) h[r].template load<PB, CPH>(rW, hcol[r], hcol1[r]);
         const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kk * cpl), crange);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) m[c] = PR::ld(rC, colc, c * PB);
+        for (int c = 0; c < 6; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
     };
     if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
     // PML slab of the box: per-lane profile values on the x/y axes, aux positions
@@ -306,14 +347,14 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         // requests for later iterations: the plane entering the queue and, with prefetch, the
         // next plane's halo and metric/medium pairs (clamped at the grid end; unused there)
         W9<S> nq;
-        nq.template load<PB>(make_rsrc((const S*)(Wb + (long long)min(max(k + 4 * FB3, 0), kmax) * wpl), wrange), col,
+        nq.template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)min(max(k + 4 * FB3, 0), kmax) * wpl), wrange), col,
                              col1);
         if constexpr (PREFETCH) fetch_plane(min(max(k + FB3, 0), kmax), hvN, MN);
         // RK registers of this plane: needed only by the final update
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         W9<S> ra, rt;
-        if constexpr (IRK == 1 || IRK == 2) ra.template load<PB>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
-        if constexpr (IRK >= 1) rt.template load<PB>(rT, col, col1);
+        if constexpr (IRK == 1 || IRK == 2) ra.template load<PB, CPS>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
+        if constexpr (IRK >= 1) rt.template load<PB, CPS>(rT, col, col1);
         // auxiliaries of this plane (lanes outside the box read element 0)
         [[maybe_unused]] float xa[9], xm[9], xt[9];
         [[maybe_unused]] unsigned xoff = 0;
@@ -325,14 +366,14 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             const int kk = min(max(kc - pK0, 0), pKn - 1);
             xoff = inside ? (unsigned)((pBase + (long long)kk * pKs) * 4) : 0u;
 #pragma unroll
-            for (int v = 0; v < 9; ++v) xa[v] = bld<float>(rXa, xoff, v * xn4);
+            for (int v = 0; v < 9; ++v) xa[v] = Pair<float>::template ld1<CPX>(rXa, xoff, v * xn4);
             if constexpr (IRK == 1 || IRK == 2) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xm[v] = bld<float>(rXm, xoff, v * xn4);
+                for (int v = 0; v < 9; ++v) xm[v] = Pair<float>::template ld1<CPX>(rXm, xoff, v * xn4);
             }
             if constexpr (IRK >= 1) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xt[v] = bld<float>(rXt, xoff, v * xn4);
+                for (int v = 0; v < 9; ++v) xt[v] = Pair<float>::template ld1<CPX>(rXt, xoff, v * xn4);
             }
             if constexpr (AXIS == 2) {
                 zBeta = p.pb[2][kc];
@@ -414,17 +455,17 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 for (int v = 0; v < 9; ++v) {
                     const float hav = ha[v] - ad * xa[v] * DT;
                     if constexpr (IRK == 0) {
-                        bst<float>(rXm, xoff, v * xn4, xa[v]);
-                        bst<float>(rXt, xoff, v * xn4, xa[v] + beta1 * hav);
-                        bst<float>(rXa, xoff, v * xn4, xa[v] + alpha2 * hav);
+                        Pair<float>::template st1<CPX>(rXm, xoff, v * xn4, xa[v]);
+                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xa[v] + beta1 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xa[v] + alpha2 * hav);
                     } else if constexpr (IRK == 1) {
-                        bst<float>(rXt, xoff, v * xn4, xt[v] + beta2 * hav);
-                        bst<float>(rXa, xoff, v * xn4, xm[v] + alpha3 * hav);
+                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xt[v] + beta2 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xm[v] + alpha3 * hav);
                     } else if constexpr (IRK == 2) {
-                        bst<float>(rXt, xoff, v * xn4, xt[v] + beta3 * hav);
-                        bst<float>(rXa, xoff, v * xn4, xm[v] + hav);
+                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xt[v] + beta3 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xm[v] + hav);
                     } else {
-                        bst<float>(rXa, xoff, v * xn4, xt[v] + beta4 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xt[v] + beta4 * hav);
                     }
                 }
             }
@@ -446,8 +487,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                     o[v] = rt[v] + beta4 * h[v];
                 }
             }
-            store9<S, PB>(rO, col, col1, o);
-            if constexpr (IRK <= 2) store9<S, PB>(rT, col, col1, t);
+            store9<S, PB, CPO>(rO, col, col1, o);
+            if constexpr (IRK <= 2) store9<S, PB, CPS>(rT, col, col1, t);
         }
         // the slot of the oldest plane receives the new one
         q0 = nq;
