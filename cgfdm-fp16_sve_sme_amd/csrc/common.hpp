@@ -41,12 +41,18 @@ template <> struct SV<__bf16> {
 // (plane = px*ny_). The bandwidth probe of this access mix served contiguous per-block runs
 // ~25 % faster than one stream per variable plane.
 //
-// The metric/medium array (13 components) stores the reference components in the slot order
-// 0-8 metrics, 9 mu, 10 lambda, 11 buoyancy, 12 J, so that the 12 values a stage reads (all but
-// J) are exactly the six pairs; atc() takes the reference numbering (0-8 metrics, 9 J, 10 mu,
-// 11 lambda, 12 buoyancy).
+// The metric/medium array (13 components) stores the reference components (0-8 metrics xi_x,
+// xi_y, xi_z, eta_x, eta_y, eta_z, zeta_x, zeta_y, zeta_z, 9 J, 10 mu, 11 lambda, 12 buoyancy)
+// in the slot order xi_x, eta_y, zeta_x, zeta_y, zeta_z, mu, lambda, buoyancy, xi_y, xi_z, eta_x,
+// eta_z, J. The grids of this library are terrain-following (x depends on i only, y on j only),
+// so x_eta = x_zeta = y_xi = y_zeta = 0 exactly (the averaged forward/backward differences of a
+// coordinate constant along an axis cancel bit for bit) and with them xi_y, xi_z, eta_x, eta_z,
+// whose adjugate terms are products of those zeros: the tile kernels read only the first four
+// pairs. atc() takes the reference numbering.
 constexpr int kBlk = 64;
-__host__ __device__ __forceinline__ constexpr int cjm_slot(int c) { return c < 9 ? c : (c == 9 ? 12 : c - 1); }
+__host__ __device__ __forceinline__ constexpr int cjm_slot(int c) {
+    return c == 0 ? 0 : c == 1 ? 8 : c == 2 ? 9 : c == 3 ? 10 : c == 4 ? 1 : c == 5 ? 11 : c == 9 ? 12 : c <= 8 ? c - 4 : c - 5;
+}
 struct Layout {
     int nx_, ny_, nz_;  // padded sizes (interior + 2*HALO)
     int px, lead, nb;   // row pitch, column shift (elements), blocks per row

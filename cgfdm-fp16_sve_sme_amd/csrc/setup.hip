@@ -10,6 +10,7 @@
 namespace cg {
 
 __device__ __forceinline__ void coord_at(const SetupParams& q, int i, int j, int k, double c[3]) {
+#pragma clang fp contract(off)
     c[0] = (i - HALO) * q.DH - q.centerX * q.DH;
     c[1] = (j - HALO) * q.DH - q.centerY * q.DH;
     if (q.use_terrain) {
@@ -23,8 +24,13 @@ __device__ __forceinline__ void coord_at(const SetupParams& q, int i, int j, int
 }
 
 // Averaged forward/backward MacCormack derivative of the coordinates and the contravariant
-// metric as the adjugate over J (reference solve_con_jac, contravariant.cpp:71-171).
+// metric as the adjugate over J (reference solve_con_jac, contravariant.cpp:71-171). Evaluated
+// without multiply-add contraction, as the reference's CPU arithmetic: the forward and backward
+// derivatives of a coordinate that is constant along an axis then cancel bit for bit, so
+// x_eta, x_zeta, y_xi, y_zeta and the metric terms built from them (xi_y, xi_z, eta_x, eta_z) are
+// exactly zero on these terrain-following grids, which the stage tile kernels rely on.
 __device__ void metric_at(const SetupParams& q, int i, int j, int k, double m[9], double& J) {
+#pragma clang fp contract(off)
     const double rDH = 1.0 / q.DH;
     const double a[5] = {(double)af_1, (double)af0, (double)af1, (double)af2, (double)af3};
     double der[3][3];  // der[var][dir]

@@ -263,19 +263,21 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const float Crho = p.Crho;
     int k = k0;
 
-    // halo cells and metric/medium pairs of a plane (the metric array's six pairs are
-    // xi_x..zt_z, mu, lambda, buoyancy)
+    // halo cells and metric/medium pairs of a plane: the metric array's first four pairs are
+    // (xi_x, eta_y), (zeta_x, zeta_y), (zeta_z, mu), (lambda, buoyancy); xi_y, xi_z, eta_x and
+    // eta_z are exactly zero on the terrain-following grids this library builds, so their terms
+    // are left out
     W9<S> hv[HPT];
-    P2 M[6];
+    P2 M[4];
     [[maybe_unused]] W9<S> hvN[HPT];
-    [[maybe_unused]] P2 MN[6];
-    auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[6]) {
+    [[maybe_unused]] P2 MN[4];
+    auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[4]) {
         const Rsrc rW = make_rsrc((const S*)(Wb + (long long)kk * wpl), wrange);
 #pragma unroll
         for (int r = 0; r < HPT; ++r) h[r].template load<PB, CPH>(rW, hcol[r], hcol1[r]);
         const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kk * cpl), crange);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
+        for (int c = 0; c < 4; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
     };
     if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
     // PML slab of the box: per-lane profile values on the x/y axes, aux positions
@@ -382,10 +384,13 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             }
         }
         if (inside && active) {
-            float Mf[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) Mf[c] = (c & 1) ? PR::hi(M[c >> 1]) : PR::lo(M[c >> 1]);
-            const float mu = PR::hi(M[4]) * DT, lam = PR::lo(M[5]) * DT, bu = PR::hi(M[5]) * (Crho * DT);
+            const float xix = PR::lo(M[0]), ety = PR::hi(M[0]);
+            const float ztx = PR::lo(M[1]), zty = PR::hi(M[1]), ztz = PR::lo(M[2]);
+            const float mu = PR::hi(M[2]) * DT, lam = PR::lo(M[3]) * DT, bu = PR::hi(M[3]) * (Crho * DT);
+            // physical derivative d/dx_c from the three one-sided ones
+            auto phys = [&](int c, float d0, float d1, float d2) {
+                return c == 0 ? xix * d0 + ztx * d2 : c == 1 ? ety * d1 + zty * d2 : ztz * d2;
+            };
             const int ox0 = lc - FB1, ox2 = lc + FB1, ox3 = lc + 2 * FB1, ox4 = lc + 3 * FB1;
             const int oy0 = lc - FB2 * LX, oy2 = lc + FB2 * LX, oy3 = lc + 2 * FB2 * LX, oy4 = lc + 3 * FB2 * LX;
             // The 27 one-sided derivatives are folded into the physical velocity gradient
@@ -406,14 +411,14 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 if constexpr (AXIS == 2) { draw[v] = d2; d2 *= zBeta; }
                 if (v < 3) {
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) G[v][cc] = Mf[cc] * d0 + Mf[3 + cc] * d1 + Mf[6 + cc] * d2;
+                    for (int cc = 0; cc < 3; ++cc) G[v][cc] = phys(cc, d0, d1, d2);
                 } else {
                     // stress component (row r0, column c0) contributes d/dx_c0 to dv[r0] and, off the
                     // diagonal, d/dx_r0 to dv[c0]
                     const int r0 = v == 3 ? 0 : v == 4 ? 1 : v == 5 ? 2 : v == 6 ? 0 : v == 7 ? 0 : 1;
                     const int c0 = v == 3 ? 0 : v == 4 ? 1 : v == 5 ? 2 : v == 6 ? 1 : v == 7 ? 2 : 2;
-                    dv[r0] += Mf[c0] * d0 + Mf[3 + c0] * d1 + Mf[6 + c0] * d2;
-                    if (v >= 6) dv[c0] += Mf[r0] * d0 + Mf[3 + r0] * d1 + Mf[6 + r0] * d2;
+                    dv[r0] += phys(c0, d0, d1, d2);
+                    if (v >= 6) dv[c0] += phys(r0, d0, d1, d2);
                 }
             }
             float h[9];
@@ -436,7 +441,10 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 // (with d, no beta) of the slab normal
 #pragma unroll
                 for (int v = 0; v < 9; ++v) h[v] += -beta * xa[v] * DT;
-                const float mx = Mf[3 * AXIS], my = Mf[3 * AXIS + 1], mz = Mf[3 * AXIS + 2];
+                // metric row of the slab normal (zero entries as above)
+                const float mx = AXIS == 0 ? xix : AXIS == 2 ? ztx : 0.f;
+                const float my = AXIS == 1 ? ety : AXIS == 2 ? zty : 0.f;
+                const float mz = AXIS == 2 ? ztz : 0.f;
                 float b[9];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) b[v] = draw[v] * dd;
@@ -496,7 +504,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
             for (int r = 0; r < HPT; ++r) hv[r] = hvN[r];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) M[c] = MN[c];
+            for (int c = 0; c < 4; ++c) M[c] = MN[c];
         }
         k += FB3;
     };

@@ -35,10 +35,13 @@ def test_cjm_flat_and_terrain(gpu, oracle_lib):
         g, o = _pair(cg, po, d, po.F64, terrain)
         a = g.cjm().astype(np.float64)
         b = o.cjm64().astype(np.float32).astype(np.float64)
+        # terrain-following grids: xi_y, xi_z, eta_x, eta_z are exactly zero on both sides (the
+        # stage tile kernels leave their terms out)
+        for c in (1, 2, 3, 5):
+            assert not a[c].any() and not b[c].any(), f"terrain={use_terrain}: metric component {c} not zero"
         # metrics, J, medium: FP64 on both sides, rounded once to FP32
         err = np.abs(a - b)
         # one FP32 rounding of each value, plus a floor at 1e-12 of the component's scale
-        # (flat-grid off-diagonal metrics are 0 on one side and ~1e-20 on the other)
         cmax = np.abs(b).reshape(13, -1).max(1)
         cmax[:9] = cmax[:9].max()  # the nine metric components share one scale
         scale = cmax[:, None, None, None]

@@ -153,6 +153,31 @@ def test_interior_rhs_is_textbook_elastodynamics(cg, oracle_lib, terrain_kind, F
         np.testing.assert_allclose(h, ref, rtol=0, atol=1e-9 * scale)
 
 
+@pytest.mark.parametrize("terrain_kind", ["flat", "hill", "rough"])
+def test_metric_off_diagonals_vanish_on_terrain_following_grids(cg, oracle_lib, terrain_kind):
+    """x depends on i only and y on j only, so the averaged forward/backward derivatives
+    x_eta, x_zeta, y_xi, y_zeta cancel bit for bit and xi_y, xi_z, eta_x, eta_z (reference
+    components 1, 2, 3, 5, built from products with them) are exactly zero; xi_x, eta_y,
+    zeta_z are not. The stage tile kernels leave the zero terms out."""
+    po = oracle_lib
+    NX, NY, NZ = 22, 20, 14
+    d = _desc(cg, NX, NY, NZ, pml=0, free_surface=1, use_terrain=int(terrain_kind != "flat"), depth=1400.0)
+    terrain = None
+    if terrain_kind == "hill":
+        terrain = models.gauss_hill(NX, NY, 100.0, h0=300.0, width=700.0)
+    elif terrain_kind == "rough":
+        terrain = np.random.default_rng(5).normal(0.0, 80.0, (NY + 6, NX + 6))
+    o = po.Oracle(d, F64)
+    o.set_model(terrain, *models.homogeneous(NX, NY, NZ))
+    c = o.cjm64()
+    for comp in (1, 2, 3, 5):
+        assert not c[comp].any(), comp
+    for comp in (0, 4, 8):
+        assert np.abs(c[comp][3:-3, 3:-3, 3:-3]).min() > 0.5, comp
+    if terrain is not None:
+        assert np.abs(c[6]).max() > 1e-3 and np.abs(c[7]).max() > 1e-3
+
+
 # ---------------------------------------------------------------- free surface
 @pytest.mark.parametrize("terrain_kind", ["flat", "tilted"])
 def test_free_surface_traction_rate_vanishes(cg, oracle_lib, terrain_kind):
