@@ -13,6 +13,11 @@ constexpr float af_1 = -0.30874f, af0 = -0.6326f, af1 = 1.2330f, af2 = -0.3334f,
 constexpr float alpha2 = 0.5f, alpha3 = 0.5f;
 constexpr float beta1 = 0.16666667f, beta2 = 0.33333333f, beta3 = 0.33333333f, beta4 = 0.16666667f;
 constexpr float Cf1 = -1.16666667f, Cf2 = 1.33333333f, Cf3 = -0.16666667f;
+// The RK accumulator after stage 0, t = m + beta1*h0, is not stored: stage 1 rebuilds it from
+// the stage-0 output w1 = m + alpha2*h0 it reads anyway, t = m + (w1 - m)*kT0 with
+// kT0 = beta1/alpha2 (exact in float). 18 of the 192 values a point moves per step are saved;
+// t carries the rounding of w1 instead of its own (one rounding either way).
+constexpr float kT0 = beta1 / alpha2;
 constexpr float PIf = 3.141592657f;
 
 // Storage codecs: values live in HBM as S and are widened to FP32 for every operation.

@@ -356,7 +356,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         W9<S> ra, rt;
         if constexpr (IRK == 1 || IRK == 2) ra.template load<PB, CPS>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
-        if constexpr (IRK >= 1) rt.template load<PB, CPS>(rT, col, col1);
+        if constexpr (IRK >= 2) rt.template load<PB, CPS>(rT, col, col1);
         // auxiliaries of this plane (lanes outside the box read element 0)
         [[maybe_unused]] float xa[9], xm[9], xt[9];
         [[maybe_unused]] unsigned xoff = 0;
@@ -477,17 +477,19 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                     }
                 }
             }
-            // RK registers (wave_rk0..3)
+            // RK registers (wave_rk0..3); the accumulator after stage 0 is rebuilt in stage 1
+            // from this stage's input (kT0)
             const Rsrc rO = make_rsrc((const S*)(Ob + pw), wrange);
             float o[9], t[9];
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
+                // the stage input is taken again from the packed queue slot (q1), so the FP32
+                // copies of the centre values need not live through the derivative phase
                 if constexpr (IRK == 0) {
-                    o[v] = cen[v] + alpha2 * h[v];
-                    t[v] = cen[v] + beta1 * h[v];
+                    o[v] = q1[v] + alpha2 * h[v];
                 } else if constexpr (IRK == 1) {
                     o[v] = ra[v] + alpha3 * h[v];
-                    t[v] = rt[v] + beta2 * h[v];
+                    t[v] = (ra[v] + (q1[v] - ra[v]) * kT0) + beta2 * h[v];
                 } else if constexpr (IRK == 2) {
                     o[v] = ra[v] + h[v];
                     t[v] = rt[v] + beta3 * h[v];
@@ -496,7 +498,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 }
             }
             store9<S, PB, CPO>(rO, col, col1, o);
-            if constexpr (IRK <= 2) store9<S, PB, CPS>(rT, col, col1, t);
+            if constexpr (IRK == 1 || IRK == 2) store9<S, PB, CPS>(rT, col, col1, t);
         }
         // the slot of the oldest plane receives the new one
         q0 = nq;

@@ -29,11 +29,13 @@ __device__ __forceinline__ float LJT(const float* J_T, int FB, float rDH) {
            (af_1 * J_T[3 - FB] + af0 * J_T[3] + af1 * J_T[3 + FB] + af2 * J_T[3 + 2 * FB] + af3 * J_T[3 + 3 * FB]) * rDH;
 }
 
-// RK register update of the 9 wavefield variables at (i, j, k) (wave_rk0..3).
+// RK register update of the 9 wavefield variables at (i, j, k) (wave_rk0..3). The accumulator
+// after stage 0 is not stored; stage 1 rebuilds it from its input (kT0).
 template <class S>
 __device__ __forceinline__ void rk_update(const StageParams& p, int i, int j, int k, const float* h) {
     S* __restrict__ Wo = (S*)p.Wout;
     const S* __restrict__ A = (const S*)p.A;
+    const S* __restrict__ Wi = (const S*)p.Win;
     S* __restrict__ T = (S*)p.T;
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
@@ -41,11 +43,10 @@ __device__ __forceinline__ void rk_update(const StageParams& p, int i, int j, in
         if (p.irk == 0) {
             float m = SV<S>::ld(A[q]);
             Wo[q] = SV<S>::st(m + alpha2 * h[v]);
-            T[q] = SV<S>::st(m + beta1 * h[v]);
         } else if (p.irk == 1) {
-            float m = SV<S>::ld(A[q]), t = SV<S>::ld(T[q]);
+            float m = SV<S>::ld(A[q]), w = SV<S>::ld(Wi[q]);
             Wo[q] = SV<S>::st(m + alpha3 * h[v]);
-            T[q] = SV<S>::st(t + beta2 * h[v]);
+            T[q] = SV<S>::st((m + (w - m) * kT0) + beta2 * h[v]);
         } else if (p.irk == 2) {
             float m = SV<S>::ld(A[q]), t = SV<S>::ld(T[q]);
             Wo[q] = SV<S>::st(m + h[v]);
