@@ -135,7 +135,7 @@ def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=5):
                       f"{steps} timed steps, OpenMP over k on {threads} threads"}
 
 
-def serving_run(reqs, max_domains, steps_per_launch, paced=True):
+def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True):
     """Serves `reqs` [(arrival_s, request)] with one cgfdm.Server; requests are submitted at
     their arrival times by a second thread (paced) or all at once. Returns latencies (s, from
     submission to results on the host), the makespan and the server counters."""
@@ -146,7 +146,7 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True):
         descs.append(cgfdm.make_desc(NX, NY, NZ, DH=r["DH"], DT=r["DT"], NT=r["NT"], nPML=r["nPML"],
                                      precision=r["precision"]))
     srv = cgfdm.Server(precision=reqs[0][1]["precision"], max_domains=max_domains,
-                       steps_per_launch=steps_per_launch, use_graph=True)
+                       steps_per_launch=steps_per_launch, use_graph=use_graph)
     ids = {}
     err = []
 
@@ -192,12 +192,12 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True):
     return np.array(lat), last - first, wall, stats
 
 
-def serving(n, seed, max_domains, steps_per_launch):
+def serving(n, seed, max_domains, steps_per_launch, use_graph=True):
     reqs = wl.serving_stream(n, seed)
     lups = sum(r["grid"][0] * r["grid"][1] * r["grid"][2] * r["NT"] for _, r in reqs)
     out = {}
     for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
-        lat, span, wall, st = serving_run(reqs, md, steps_per_launch)
+        lat, span, wall, st = serving_run(reqs, md, steps_per_launch, use_graph=use_graph)
         out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
                      "glups": round(lups / span / 1e9, 3), "peak_resident": st["peak_resident"],
@@ -249,6 +249,7 @@ def main():
                     help="requests of the serving stream to run (0: skip; 512: the whole stream)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-steps-per-launch", type=int, default=8)
+    ap.add_argument("--serve-graph", type=int, default=1, help="1: replay captured HIP graphs, 0: launch directly")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -271,7 +272,7 @@ def main():
         for prec in precs:
             res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
         del model
-    serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch) \
+    serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph)) \
         if args.serve_requests > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

@@ -373,7 +373,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
                 for (int v = 0; v < 9; ++v) xm[v] = Pair<float>::template ld1<CPX>(rXm, xoff, v * xn4);
             }
-            if constexpr (IRK >= 1) {
+            if constexpr (IRK >= 2) {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) xt[v] = Pair<float>::template ld1<CPX>(rXt, xoff, v * xn4);
             }
@@ -462,12 +462,13 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
                     const float hav = ha[v] - ad * xa[v] * DT;
+                    // as for the wavefield, the accumulator after stage 0 is rebuilt in stage 1
                     if constexpr (IRK == 0) {
                         Pair<float>::template st1<CPX>(rXm, xoff, v * xn4, xa[v]);
-                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xa[v] + beta1 * hav);
                         Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xa[v] + alpha2 * hav);
                     } else if constexpr (IRK == 1) {
-                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xt[v] + beta2 * hav);
+                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4,
+                                                       (xm[v] + (xa[v] - xm[v]) * kT0) + beta2 * hav);
                         Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xm[v] + alpha3 * hav);
                     } else if constexpr (IRK == 2) {
                         Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xt[v] + beta3 * hav);

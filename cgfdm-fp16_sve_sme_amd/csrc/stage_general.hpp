@@ -261,18 +261,19 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
                 ha[7] += ((zz * Vz0 + zx * Vz2) * mu) * DT;
                 ha[8] += ((zz * Vz1 + zy * Vz2) * mu) * DT;
             }
-            // wave_pml_rk0..3 (pml_rk.cpp:14-70), in place: each point owns its aux values
+            // wave_pml_rk0..3 (pml_rk.cpp:14-70), in place: each point owns its aux values; the
+            // accumulator after stage 0 is rebuilt in stage 1 (kT0)
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const long long q = v * X.n + pos;
                 if (p.irk == 0) {
                     const float m = aold[v];
                     X.m[q] = m;
-                    X.t[q] = m + beta1 * ha[v];
                     X.a[q] = m + alpha2 * ha[v];
                 } else if (p.irk == 1) {
-                    X.t[q] = X.t[q] + beta2 * ha[v];
-                    X.a[q] = X.m[q] + alpha3 * ha[v];
+                    const float m = X.m[q];
+                    X.t[q] = (m + (aold[v] - m) * kT0) + beta2 * ha[v];
+                    X.a[q] = m + alpha3 * ha[v];
                 } else if (p.irk == 2) {
                     X.t[q] = X.t[q] + beta3 * ha[v];
                     X.a[q] = X.m[q] + ha[v];
