@@ -481,6 +481,13 @@ int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) 
     cgfdm_domain* h = new cgfdm_domain();
     hipError_t e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) { set_err("hipStreamCreate: %s", hipGetErrorString(e)); delete h; return CGFDM_EHIP; }
+    if ((e = hipStreamCreate(&h->side)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming)) != hipSuccess) {
+        set_err("side stream: %s", hipGetErrorString(e));
+        cgfdm_domain_destroy(h);
+        return CGFDM_EHIP;
+    }
     if ((rc = domain_init(h, desc, device))) {
         cgfdm_domain_destroy(h);
         return rc;
@@ -496,6 +503,12 @@ int cgfdm_domain_destroy(cgfdm_domain* h) {
     for (void* p : h->allocs) (void)hipFree(p);
     for (auto e : h->evStart) (void)hipEventDestroy(e);
     for (auto e : h->evStop) (void)hipEventDestroy(e);
+    if (h->side) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+    }
+    if (h->evFork) (void)hipEventDestroy(h->evFork);
+    if (h->evJoin) (void)hipEventDestroy(h->evJoin);
     if (h->stream && h->ownStream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CGFDM_OK;
@@ -682,7 +695,9 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
                     h->evUsed = 0;
                 }
             } else {
-                LCHK(launch_stage(h->prec, p, 1, h->stream, nullptr));
+                // untimed: the boundary kernels run beside the bulk kernel on the side stream
+                const StageFork fk{h->side, h->evFork, h->evJoin};
+                LCHK(launch_stage(h->prec, p, 1, h->stream, nullptr, h->side ? &fk : nullptr));
             }
             FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;
         }
@@ -844,7 +859,11 @@ int cgfdm_decompose(const cgfdm_desc* desc, int* boxes, int max_boxes) {
         const int B[6] = {q.bi0, q.bi1, q.bj0, q.bj1, q.bk0, q.bk1};
         put(0, -1, B);
     }
-    for (int b = 0; b < q.npbox; ++b) put(1, q.pbox[b][7], q.pbox[b]);
+    for (int b = 0; b < q.npbox; ++b) {
+        const int mask = q.pbox[b][6];
+        if (mask == 1 || mask == 2 || mask == 4) put(1, mask == 1 ? 0 : mask == 2 ? 1 : 2, q.pbox[b]);
+        else put(4, mask, q.pbox[b]);
+    }
     if (q.fs_on) {
         const int B[6] = {q.bi0, q.bi1, q.bj0, q.bj1, q.fk0, q.fk1};
         put(3, -1, B);
@@ -858,7 +877,7 @@ int cgfdm_partition(const cgfdm_domain* h, long long* out) {
     StageParams q = domain_stage_params(h);
     stage_boxes(q, 1);
     out[0] = q.bulk_on ? (long long)(q.bi1 - q.bi0) * (q.bj1 - q.bj0) * (q.bk1 - q.bk0) : 0;
-    out[1] = q.boxStart[q.nbox] + (q.fs_on ? (long long)(q.bi1 - q.bi0) * (q.bj1 - q.bj0) * (q.fk1 - q.fk0) : 0);
+    out[1] = (long long)q.nx * q.ny * q.nz - out[0];
     out[2] = h->bulk_cfg;
     out[3] = h->bulk_kc;
     return CGFDM_OK;

@@ -69,7 +69,7 @@ __global__ void k_clear_slot(BatchDev b, int s) {
 __global__ void k_build_tables(BatchDev b) {
     if (threadIdx.x != 0) return;
     int nb = 0, ng = 0, tiles = 0, nf = 0, ftiles = 0;
-    int np[3] = {0, 0, 0}, ptiles[3] = {0, 0, 0};
+    int np[kNumPmlMasks] = {}, ptiles[kNumPmlMasks] = {};
     long long chunks = 0;
     for (int s = 0; s < b.maxSlots; ++s) {
         const SlotDev& sd = b.slots[s];
@@ -89,15 +89,14 @@ __global__ void k_build_tables(BatchDev b) {
             e.slot = s; e.box = x; e.start = chunks;
             chunks += sd.boxChunks[x];
         }
-        for (int a = 0; a < 3; ++a) {
-            const PmlTiling& t = sd.pt[a];
-            for (int x = 0; x < t.n; ++x) {
-                PmlEnt& e = b.tab->pml[a][np[a]++];
+        const PmlTiles& t = sd.pt;
+        for (int g = 0; g < kNumPmlMasks; ++g)
+            for (int x = t.first[g]; x < t.first[g + 1]; ++x) {
+                PmlEnt& e = b.tab->pml[g][np[g]++];
                 e.slot = s; e.box = t.box[x]; e.gx = t.gx[x]; e.gy = t.gy[x]; e.nch = t.nch[x]; e.kc = t.kc[x];
-                e.start = ptiles[a];
-                ptiles[a] += t.gx[x] * t.gy[x] * t.nch[x];
+                e.start = ptiles[g];
+                ptiles[g] += t.gx[x] * t.gy[x] * t.nch[x];
             }
-        }
     }
     b.tab->nBulk = nb;
     b.tab->nTiles = tiles;
@@ -105,9 +104,9 @@ __global__ void k_build_tables(BatchDev b) {
     b.tab->nChunks = chunks;
     b.tab->nFs = nf;
     b.tab->nFsTiles = ftiles;
-    for (int a = 0; a < 3; ++a) {
-        b.tab->nPml[a] = np[a];
-        b.tab->nPmlTiles[a] = ptiles[a];
+    for (int g = 0; g < kNumPmlMasks; ++g) {
+        b.tab->nPml[g] = np[g];
+        b.tab->nPmlTiles[g] = ptiles[g];
     }
 }
 
@@ -154,13 +153,14 @@ __global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b) {
     }
 }
 
-template <class S, int TX, int TY, int IRK, int AXIS>
-__global__ void __launch_bounds__(TX* TY) k_pml_multi(BatchDev b) {
+template <class S, int IRK, int MASK>
+__global__ void __launch_bounds__(pml_tx(MASK) * pml_ty(MASK)) k_pml_multi(BatchDev b) {
+    constexpr int TX = pml_tx(MASK), TY = pml_ty(MASK), g = pml_group_of(MASK);
     __shared__ BulkLds<TX, TY> lds;
-    const int nEnt = b.tab->nPml[AXIS], nTiles = b.tab->nPmlTiles[AXIS];
+    const int nEnt = b.tab->nPml[g], nTiles = b.tab->nPmlTiles[g];
     for (int w = blockIdx.x; w < nTiles; w += gridDim.x) {
         const int t = (int)xcd_order(w, nTiles);
-        const PmlEnt& e = b.tab->pml[AXIS][find_entry(b.tab->pml[AXIS], nEnt, t)];
+        const PmlEnt& e = b.tab->pml[g][find_entry(b.tab->pml[g], nEnt, t)];
         const SlotDev& sd = b.slots[e.slot];
         if (sd.it >= sd.NT) continue;
         const StageParams& p = b.sp[e.slot * 4 + IRK];
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(TX* TY) k_pml_multi(BatchDev b) {
         const int* B = p.pbox[e.box];
         const int kb = B[4] + ch * e.kc;
         const int ke = min(kb + e.kc, B[5]);
-        bulk_tile<S, TX, TY, IRK, AXIS>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[6], bx, by, kb, ke, lds.v);
+        bulk_tile<S, TX, TY, IRK, MASK>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[7], bx, by, kb, ke, lds.v);
         __syncthreads();
     }
 }
@@ -249,43 +249,60 @@ int batch_clear_slot(const BatchDev& b, int s, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+// One RK stage of every resident domain: the bulk tiles on st, the PML, free-surface and general
+// work on the side stream of fk (if given) concurrently, joined back into st.
 template <class S, int TX, int TY, int IRK>
-static void batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+static int batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk) {
+    hipStream_t sb = st;
+    if (fk && fk->side) {
+        if (hipEventRecord(fk->fork, st) != hipSuccess || hipStreamWaitEvent(fk->side, fk->fork, 0) != hipSuccess)
+            return -3;
+        sb = fk->side;
+    }
     k_bulk_multi<S, TX, TY, IRK><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_pml_multi<S, 16, 16, IRK, 0><<<g.pmlBlocks[0], 256, 0, st>>>(b);
-    k_pml_multi<S, 64, 4, IRK, 1><<<g.pmlBlocks[1], 256, 0, st>>>(b);
-    k_pml_multi<S, 64, 4, IRK, 2><<<g.pmlBlocks[2], 256, 0, st>>>(b);
-    k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, st>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, st>>>(b, IRK);
+    k_pml_multi<S, IRK, 1><<<g.pmlBlocks[0], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 2><<<g.pmlBlocks[1], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 4><<<g.pmlBlocks[2], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 3><<<g.pmlBlocks[3], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 5><<<g.pmlBlocks[4], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 6><<<g.pmlBlocks[5], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 7><<<g.pmlBlocks[6], 256, 0, sb>>>(b);
+    k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, sb>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, sb>>>(b, IRK);
+    if (sb != st) {
+        if (hipEventRecord(fk->join, sb) != hipSuccess || hipStreamWaitEvent(st, fk->join, 0) != hipSuccess)
+            return -3;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 template <class S, int TX, int TY>
-static int batch_step_t(const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+static int batch_step_t(const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk) {
     k_source_multi<S><<<dim3(2, b.maxSlots), 256, 0, st>>>(b);
-    batch_stage<S, TX, TY, 0>(b, g, st);
-    batch_stage<S, TX, TY, 1>(b, g, st);
-    batch_stage<S, TX, TY, 2>(b, g, st);
-    batch_stage<S, TX, TY, 3>(b, g, st);
+    if (batch_stage<S, TX, TY, 0>(b, g, st, fk) || batch_stage<S, TX, TY, 1>(b, g, st, fk) ||
+        batch_stage<S, TX, TY, 2>(b, g, st, fk) || batch_stage<S, TX, TY, 3>(b, g, st, fk))
+        return -3;
     k_record_multi<S><<<dim3(16, b.maxSlots), 256, 0, st>>>(b);
     k_advance<<<(b.maxSlots + 63) / 64, 64, 0, st>>>(b);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st) {
+int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk) {
     switch (prec * 2 + (cfg ? 1 : 0)) {
-        case P_F32 * 2: return batch_step_t<float, 64, 8>(b, g, st);
-        case P_F32 * 2 + 1: return batch_step_t<float, 64, 4>(b, g, st);
-        case P_F16 * 2: return batch_step_t<_Float16, 64, 8>(b, g, st);
-        case P_F16 * 2 + 1: return batch_step_t<_Float16, 64, 4>(b, g, st);
-        case P_BF16 * 2: return batch_step_t<__bf16, 64, 8>(b, g, st);
-        case P_BF16 * 2 + 1: return batch_step_t<__bf16, 64, 4>(b, g, st);
+        case P_F32 * 2: return batch_step_t<float, 64, 8>(b, g, st, fk);
+        case P_F32 * 2 + 1: return batch_step_t<float, 64, 4>(b, g, st, fk);
+        case P_F16 * 2: return batch_step_t<_Float16, 64, 8>(b, g, st, fk);
+        case P_F16 * 2 + 1: return batch_step_t<_Float16, 64, 4>(b, g, st, fk);
+        case P_BF16 * 2: return batch_step_t<__bf16, 64, 8>(b, g, st, fk);
+        case P_BF16 * 2 + 1: return batch_step_t<__bf16, 64, 4>(b, g, st, fk);
     }
     return -1;
 }
 
 // Persistent grid sizes: as many workgroups as fit on the device at once.
 int batch_grid(int prec, int cfg, BatchGrid& g) {
-    int dev = 0, ncu = 0, nb = 0, ng = 0, n0 = 0, n1 = 0, n2 = 0, nf = 0;
+    int dev = 0, ncu = 0, nb = 0, ng = 0, nf = 0;
+    int npm[kNumPmlMasks] = {};
     if (hipGetDevice(&dev) != hipSuccess) return -3;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -3;
     hipError_t e1 = hipSuccess, e2 = hipSuccess;
@@ -294,9 +311,13 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4, 1>, bt, 0)               \
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8, 1>, bt, 0);              \
     e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);                  \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_pml_multi<S, 16, 16, 1, 0>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_pml_multi<S, 64, 4, 1, 1>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n2, k_pml_multi<S, 64, 4, 1, 2>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[0], k_pml_multi<S, 1, 1>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[1], k_pml_multi<S, 1, 2>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[2], k_pml_multi<S, 1, 4>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[3], k_pml_multi<S, 1, 3>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[4], k_pml_multi<S, 1, 5>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[5], k_pml_multi<S, 1, 6>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[6], k_pml_multi<S, 1, 7>, 256, 0); \
     if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nf, k_fs_multi<S, 1>, kFsTx * kFsTy, 0);
     switch (prec) {
         case P_F32: CG_OCC(float) break;
@@ -309,8 +330,7 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     // multiples of the XCD count (see k_bulk_multi)
     g.bulkBlocks = (ncu * (nb > 0 ? nb : 1) + kXcds - 1) / kXcds * kXcds;
     g.genBlocks = (ncu * (ng > 0 ? ng : 1) + kXcds - 1) / kXcds * kXcds;
-    const int npm[3] = {n0, n1, n2};
-    for (int a = 0; a < 3; ++a) g.pmlBlocks[a] = (ncu * (npm[a] > 0 ? npm[a] : 1) + kXcds - 1) / kXcds * kXcds;
+    for (int a = 0; a < kNumPmlMasks; ++a) g.pmlBlocks[a] = (ncu * (npm[a] > 0 ? npm[a] : 1) + kXcds - 1) / kXcds * kXcds;
     g.fsBlocks = (ncu * (nf > 0 ? nf : 1) + kXcds - 1) / kXcds * kXcds;
     return 0;
 }

@@ -17,7 +17,7 @@ struct SlotDev {
     // bulk tiling: gx x gy column tiles, nch chunks of kc planes; general: chunks per box
     int gx, gy, nch, kc;
     int fgx, fgy;       // free-surface tiles over the bulk columns (0: no free surface)
-    PmlTiling pt[3];    // single-slab PML boxes by normal axis
+    PmlTiles pt;        // PML boxes by kind
     int nbox;
     long long boxChunks[kMaxGenBoxes];
     Layout lay;
@@ -50,9 +50,9 @@ struct FsEnt { int slot, gx, gy, start; };
 struct BatchTables {
     int nBulk, nTiles, nGen, pad;
     long long nChunks;
-    int nPml[3], nPmlTiles[3];
+    int nPml[kNumPmlMasks], nPmlTiles[kNumPmlMasks];
     BulkEnt bulk[kMaxSlots];
-    PmlEnt pml[3][kMaxSlots * 2];
+    PmlEnt pml[kNumPmlMasks][kMaxSlots * 8];  // at most 8 boxes of one kind per domain
     GenEnt gen[kMaxSlots * kMaxGenBoxes];
     int nFs, nFsTiles;
     FsEnt fs[kMaxSlots];
@@ -66,13 +66,13 @@ struct BatchDev {
     int maxSlots;
 };
 
-struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[3], fsBlocks; };
+struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[kNumPmlMasks], fsBlocks; };
 
 void stage_boxes(StageParams& q, int use_fast);
-PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY);
+PmlTiles pml_tiles(const StageParams& q);
 int batch_write_slot(const BatchDev& b, int s, const SlotDev& sd, const StageParams& base, hipStream_t st);
 int batch_clear_slot(const BatchDev& b, int s, hipStream_t st);
-int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st);
+int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk);
 int batch_grid(int prec, int cfg, BatchGrid& g);
 
 }  // namespace cg

@@ -146,7 +146,18 @@ struct Slab {
     int n0, n1; // pos = i0 + j0*n0 + k0*n0*n1 with (i0,j0,k0) relative to the slab origin
 };
 
-constexpr int kMaxGenBoxes = 12;
+constexpr int kMaxGenBoxes = 4;   // the free-surface layers of the PML columns
+constexpr int kMaxPBoxes = 26;    // PML boxes: 6 faces, 12 edges, 8 corners
+
+// PML box kinds by the set of slab normals they lie in (bit 0 x, bit 1 y, bit 2 z), in launch
+// order: faces x, y, z, then edges xy, xz, yz, then corners xyz.
+constexpr int kNumPmlMasks = 7;
+__host__ __device__ __forceinline__ constexpr int pml_mask_at(int g) {
+    return g == 0 ? 1 : g == 1 ? 2 : g == 2 ? 4 : g == 3 ? 3 : g == 4 ? 5 : g == 5 ? 6 : 7;
+}
+__host__ __device__ __forceinline__ constexpr int pml_group_of(int mask) {
+    return mask == 1 ? 0 : mask == 2 ? 1 : mask == 4 ? 2 : mask == 3 ? 3 : mask == 5 ? 4 : mask == 6 ? 5 : 6;
+}
 
 // Everything one RK stage needs. Passed by value to the stage kernels.
 struct StageParams {
@@ -169,17 +180,17 @@ struct StageParams {
     // Decomposition of the interior (stage_boxes):
     //  - the bulk box [bi0,bi1)x[bj0,bj1)x[bk0,bk1): no PML slab, no free-surface layer, covered
     //    by the tile kernel;
-    //  - npbox single-slab PML boxes (the part of a slab outside every other slab and the
-    //    free-surface layers), covered by the tile kernel with that slab's auxiliary update;
+    //  - npbox PML boxes outside the free-surface layers, each inside the slabs of one set of
+    //    normals (a face: one slab; an edge: two; a corner: three), covered by the tile kernel
+    //    with those slabs' auxiliary updates;
     //  - the free-surface box [bi0,bi1)x[bj0,bj1)x[fk0,fk1): the three free-surface layers above
     //    the bulk columns (no PML), covered by the free-surface tile kernel;
-    //  - nbox general boxes, covered point by point: the edges and corners where slabs meet and
-    //    the free-surface layers of the PML columns.
+    //  - nbox general boxes, covered point by point: the free-surface layers of the PML columns.
     int bulk_on, fs_on;
     int bulk_cfg, bulk_kc;  // tile shape index and z-chunk length (0: automatic)
     int bi0, bi1, bj0, bj1, bk0, bk1, fk0, fk1;
     int npbox;
-    int pbox[6][8];         // i0,i1,j0,j1,k0,k1, slab index (0..5), axis (0..2)
+    int pbox[kMaxPBoxes][8];  // i0,i1,j0,j1,k0,k1, normals mask, slab per axis (4 bits each, 15: none)
     int nbox;
     int box[kMaxGenBoxes][6];            // i0,i1,j0,j1,k0,k1
     long long boxStart[kMaxGenBoxes + 1];  // prefix sums of box sizes
@@ -219,7 +230,13 @@ int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 // One RK stage: the bulk kernel (use_fast != 0) then the general kernel over the remaining
 // boxes (skipped when use_fast == 2, used to time the bulk kernel alone). `mid`, if given, is
 // recorded between the two launches.
-int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid);
+// One fused RK stage on stream s. With a side stream (and two events), the PML, free-surface and
+// general kernels run on it concurrently with the bulk kernel (they write disjoint points and
+// read only the stage input), joined back into s at the end; mid (if given) is recorded on s
+// after the bulk kernel.
+struct StageFork { hipStream_t side; hipEvent_t fork, join; };
+int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid,
+                 const StageFork* fk = nullptr);
 void stage_boxes(StageParams& q, int use_fast);
 
 // Point lists (sources, stations) carry padded (i, j, k) triples.

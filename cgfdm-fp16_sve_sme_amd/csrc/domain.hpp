@@ -41,6 +41,9 @@ struct cgfdm_domain {
     int dev = 0;
     hipStream_t stream = nullptr;
     bool ownStream = true;
+    // side stream of the boundary kernels and its fork/join events (own-stream domains)
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     bool async = false;       // pooled mode: no host synchronisation in set-up calls
     cg::Bump devPool, hostPool;
     bool pooled = false;

@@ -169,6 +169,10 @@ struct cgfdm_server {
     BatchGrid grid{};
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // side stream of the boundary kernels, forked from and joined into `stream` every stage
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    StageFork fork() const { return StageFork{side, evFork, evJoin}; }
     Clock::time_point t0;
 
     std::mutex mu;
@@ -252,7 +256,7 @@ int cgfdm_server::setup_resident(Resident& R) {
         sd.fgx = (base.bi1 - base.bi0 + kFsTx - 1) / kFsTx;
         sd.fgy = (base.bj1 - base.bj0 + kFsTy - 1) / kFsTy;
     }
-    for (int a = 0; a < 3; ++a) sd.pt[a] = pml_tiling(base, a, a == 0 ? 16 : 64, a == 0 ? 16 : 4);
+    sd.pt = pml_tiles(base);
     sd.nbox = base.nbox;
     for (int x = 0; x < base.nbox; ++x)
         sd.boxChunks[x] = (base.boxStart[x + 1] - base.boxStart[x] + kGenChunk - 1) / kGenChunk;
@@ -362,7 +366,10 @@ void cgfdm_server::loop() {
                 e = hipGraphLaunch(exec, stream);
             } else {
                 for (int s = 0; s < G && e == hipSuccess; ++s)
-                    if (batch_step(cfg.precision, cfgTile, b, grid, stream)) e = hipGetLastError();
+                    if (const StageFork fk = fork(); batch_step(cfg.precision, cfgTile, b, grid, stream, &fk)) {
+                        e = hipGetLastError();
+                        if (e == hipSuccess) e = hipErrorUnknown;
+                    }
             }
             if (e != hipSuccess) { fail_all(CGFDM_EHIP, std::string("batch launch: ") + hipGetErrorString(e)); return; }
         }
@@ -441,9 +448,18 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         if (S->b.sp) (void)hipFree(S->b.sp);
         if (S->b.tab) (void)hipFree(S->b.tab);
         for (auto& v : S->ev) if (v) (void)hipEventDestroy(v);
+        if (S->evFork) (void)hipEventDestroy(S->evFork);
+        if (S->evJoin) (void)hipEventDestroy(S->evJoin);
+        if (S->side) (void)hipStreamDestroy(S->side);
         (void)hipStreamDestroy(S->stream);
         return rc;
     };
+    if ((e = hipStreamCreate(&S->side)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&S->evFork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&S->evJoin, hipEventDisableTiming)) != hipSuccess) {
+        set_err("side stream: %s", hipGetErrorString(e));
+        return undo(CGFDM_EHIP);
+    }
     if ((e = hipMalloc((void**)&S->devBase, budget)) != hipSuccess) { (void)hipGetLastError(); set_err("arena of %zu bytes: %s", budget, hipGetErrorString(e)); return undo(CGFDM_ENOMEM); }
     if ((e = hipHostMalloc((void**)&S->hostBase, hostBudget, 0)) != hipSuccess) { set_err("pinned arena: %s", hipGetErrorString(e)); return undo(CGFDM_ENOMEM); }
     S->devArena.reset(budget);
@@ -468,7 +484,9 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         hipGraph_t g = nullptr;
         if ((e = hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) { set_err("capture: %s", hipGetErrorString(e)); return undo(CGFDM_EHIP); }
         int rc = 0;
-        for (int s = 0; s < c->steps_per_launch && rc == 0; ++s) rc = batch_step(c->precision, S->cfgTile, S->b, S->grid, S->stream);
+        const StageFork fk = S->fork();
+        for (int s = 0; s < c->steps_per_launch && rc == 0; ++s)
+            rc = batch_step(c->precision, S->cfgTile, S->b, S->grid, S->stream, &fk);
         e = hipStreamEndCapture(S->stream, &g);
         if (rc || e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); if (g) (void)hipGraphDestroy(g); return undo(CGFDM_EHIP); }
         e = hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0);
@@ -623,6 +641,10 @@ int cgfdm_server_destroy(cgfdm_server* s) {
     (void)hipFree(s->b.sp);
     (void)hipFree(s->b.tab);
     for (auto& v : s->ev) (void)hipEventDestroy(v);
+    (void)hipStreamSynchronize(s->side);
+    (void)hipEventDestroy(s->evFork);
+    (void)hipEventDestroy(s->evJoin);
+    (void)hipStreamDestroy(s->side);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return CGFDM_OK;

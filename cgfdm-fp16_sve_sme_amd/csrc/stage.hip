@@ -1,5 +1,6 @@
-// stage.hip — launchers of the fused RK-stage kernels: the bulk kernel over the interior box and
-// the general kernel over the PML slabs and free-surface layers around it.
+// stage.hip — launchers of the fused RK-stage kernels: the bulk kernel over the interior box, the
+// PML tile kernels over the slabs around it, the free-surface tile kernel and the general kernel
+// over the free-surface layers of the PML columns.
 #include "stage_general.hpp"
 #include "stage_bulk.hpp"
 #include "stage_fs.hpp"
@@ -45,17 +46,21 @@ static void add_box(StageParams& q, int i0, int i1, int j0, int j1, int k0, int 
     ++q.nbox;
 }
 
-static void add_pbox(StageParams& q, int i0, int i1, int j0, int j1, int k0, int k1, int slab, int axis) {
+// PML box inside the slabs sx (x normal), sy, sz (-1: none).
+static void add_pbox(StageParams& q, int i0, int i1, int j0, int j1, int k0, int k1, int sx, int sy, int sz) {
     if (i1 <= i0 || j1 <= j0 || k1 <= k0) return;
     int* B = q.pbox[q.npbox++];
-    B[0] = i0; B[1] = i1; B[2] = j0; B[3] = j1; B[4] = k0; B[5] = k1; B[6] = slab; B[7] = axis;
+    B[0] = i0; B[1] = i1; B[2] = j0; B[3] = j1; B[4] = k0; B[5] = k1;
+    B[6] = (sx >= 0 ? 1 : 0) | (sy >= 0 ? 2 : 0) | (sz >= 0 ? 4 : 0);
+    B[7] = (sx >= 0 ? sx : 15) | (sy >= 0 ? sy : 15) << 4 | (sz >= 0 ? sz : 15) << 8;
 }
 
 // Partition of the interior. Along each axis: the low slab [lo, lo+np), the middle and the high
 // slab (z: bottom slab, middle, and either the three free-surface layers or the top slab). The
-// middle^3 cell is the bulk box; a slab cell whose other two ranges are middle is a
-// single-slab PML box; all other cells (edges and corners where slabs meet, the free-surface
-// layers) are general boxes. With use_fast == 0 the whole interior is one general box.
+// middle^3 cell is the bulk box; every other cell below the free-surface layers is a PML box of
+// the slabs it lies in (faces: one slab, edges: two, corners: three); with a free surface, its
+// three layers are the free-surface box above the bulk columns and four general boxes over the
+// PML columns. With use_fast == 0 the whole interior is one general box.
 void stage_boxes(StageParams& q, int use_fast) {
     const int lo = HALO, hiI = HALO + q.nx, hiJ = HALO + q.ny, hiK = HALO + q.nz;
     const int np = q.pml ? q.nPML : 0;
@@ -77,23 +82,24 @@ void stage_boxes(StageParams& q, int use_fast) {
         return;
     }
     if (np > 0) {
-        add_pbox(q, I0, I1, J1, J2, K1, K2, 0, 0);  // x low
-        add_pbox(q, I2, hiI, J1, J2, K1, K2, 3, 0); // x high
-        add_pbox(q, I1, I2, J0, J1, K1, K2, 1, 1);  // y low
-        add_pbox(q, I1, I2, J2, hiJ, K1, K2, 4, 1); // y high
-        add_pbox(q, I1, I2, J1, J2, K0, K1, 2, 2);  // bottom
-        if (!q.fs) add_pbox(q, I1, I2, J1, J2, K2, hiK, 5, 2);  // top
+        // cells of the 3x3x3 split (the top z cell only without a free surface); slab indices
+        // x1,y1,z1,x2,y2,z2 = 0..5
+        const int xr[3][3] = {{I0, I1, 0}, {I1, I2, -1}, {I2, hiI, 3}};
+        const int yr[3][3] = {{J0, J1, 1}, {J1, J2, -1}, {J2, hiJ, 4}};
+        const int zr[3][3] = {{K0, K1, 2}, {K1, K2, -1}, {K2, hiK, 5}};
+        const int nzc = q.fs ? 2 : 3;
+        // faces first, then edges, then corners (the kinds' launch order)
+        for (int nslab = 1; nslab <= 3; ++nslab)
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 3; ++b)
+                    for (int a = 0; a < 3; ++a) {
+                        if (c >= nzc) continue;
+                        const int n = (xr[a][2] >= 0) + (yr[b][2] >= 0) + (zr[c][2] >= 0);
+                        if (n != nslab) continue;
+                        add_pbox(q, xr[a][0], xr[a][1], yr[b][0], yr[b][1], zr[c][0], zr[c][1], xr[a][2], yr[b][2],
+                                 zr[c][2]);
+                    }
     }
-    // bottom ring (z in the bottom slab, outside its single-slab part)
-    add_box(q, lo, hiI, J0, J1, K0, K1);
-    add_box(q, lo, hiI, J2, hiJ, K0, K1);
-    add_box(q, I0, I1, J1, J2, K0, K1);
-    add_box(q, I2, hiI, J1, J2, K0, K1);
-    // vertical edges where the x and y slabs meet
-    add_box(q, I0, I1, J0, J1, K1, K2);
-    add_box(q, I0, I1, J2, hiJ, K1, K2);
-    add_box(q, I2, hiI, J0, J1, K1, K2);
-    add_box(q, I2, hiI, J2, hiJ, K1, K2);
     if (q.fs) {
         // free-surface layers: above the bulk columns by the free-surface tile kernel, the ring
         // of PML columns by the general kernel
@@ -104,57 +110,69 @@ void stage_boxes(StageParams& q, int use_fast) {
         add_box(q, lo, hiI, J2, hiJ, K2, hiK);
         add_box(q, lo, I1, J1, J2, K2, hiK);
         add_box(q, I2, hiI, J1, J2, K2, hiK);
-    } else {
-        add_box(q, lo, hiI, J0, J1, K2, hiK);
-        add_box(q, lo, hiI, J2, hiJ, K2, hiK);
-        add_box(q, I0, I1, J1, J2, K2, hiK);
-        add_box(q, I2, hiI, J1, J2, K2, hiK);
     }
 }
 
-// Tiles of the PML boxes of one normal axis, about `want` workgroups in all.
-PmlTiling pml_tiling(const StageParams& q, int axis, int TX, int TY) {
-    PmlTiling t{};
+// Tiles of every PML box, grouped by kind (pml_mask_at order); z chunks sized so each kind's
+// launch has about 1024 workgroups, with chunks of at least 5 planes (the tile loop's unit):
+// the small edge and corner boxes are cut into short chunks so their launches are not one long
+// z march.
+PmlTiles pml_tiles(const StageParams& q) {
+    PmlTiles t{};
     t.start[0] = 0;
-    int cols = 0;
-    for (int b = 0; b < q.npbox; ++b) {
-        const int* B = q.pbox[b];
-        if (B[7] != axis) continue;
-        const int e = t.n++;
-        t.box[e] = b;
-        t.gx[e] = (B[1] - pml_tile_x0(q.lay, B[0], TX) + TX - 1) / TX;
-        t.gy[e] = (B[3] - B[2] + TY - 1) / TY;
-        cols += t.gx[e] * t.gy[e];
+    for (int g = 0; g < kNumPmlMasks; ++g) {
+        const int mask = pml_mask_at(g);
+        const int TX = pml_tx(mask), TY = pml_ty(mask);
+        t.first[g] = t.n;
+        int cols = 0;
+        for (int b = 0; b < q.npbox; ++b) {
+            const int* B = q.pbox[b];
+            if (B[6] != mask) continue;
+            const int e = t.n++;
+            t.box[e] = b;
+            t.gx[e] = (B[1] - pml_tile_x0(q.lay, B[0], TX) + TX - 1) / TX;
+            t.gy[e] = (B[3] - B[2] + TY - 1) / TY;
+            cols += t.gx[e] * t.gy[e];
+        }
+        const int want = 1024;
+        for (int e = t.first[g]; e < t.n; ++e) {
+            const int* B = q.pbox[t.box[e]];
+            const int nzb = B[5] - B[4];
+            const int nch = std::max(1, std::min(nzb / 5, (want + cols - 1) / std::max(cols, 1)));
+            t.kc[e] = (nzb + nch - 1) / nch;
+            t.nch[e] = (nzb + t.kc[e] - 1) / t.kc[e];
+            t.start[e + 1] = t.start[e] + t.gx[e] * t.gy[e] * t.nch[e];
+        }
     }
-    const int want = 1024;
-    for (int e = 0; e < t.n; ++e) {
-        const int* B = q.pbox[t.box[e]];
-        const int nzb = B[5] - B[4];
-        const int nch = std::max(1, std::min(nzb / 8, (want + cols - 1) / std::max(cols, 1)));
-        t.kc[e] = (nzb + nch - 1) / nch;
-        t.nch[e] = (nzb + t.kc[e] - 1) / t.kc[e];
-        t.start[e + 1] = t.start[e] + t.gx[e] * t.gy[e] * t.nch[e];
-    }
+    t.first[kNumPmlMasks] = t.n;
     return t;
 }
 
-template <class S, int TX, int TY, int AXIS>
-static void launch_pml(const StageParams& q, hipStream_t s) {
-    const PmlTiling t = pml_tiling(q, AXIS, TX, TY);
-    if (t.n == 0 || t.start[t.n] == 0) return;
-    const unsigned grid = (unsigned)t.start[t.n];
+template <class S, int MASK>
+static void launch_pml(const StageParams& q, const PmlTiles& t, hipStream_t s) {
+    constexpr int g = pml_group_of(MASK), TX = pml_tx(MASK), TY = pml_ty(MASK);
+    const int tiles = t.start[t.first[g + 1]] - t.start[t.first[g]];
+    if (tiles <= 0) return;
+    const unsigned grid = (unsigned)tiles;
     switch (q.irk) {
-        case 0: k_stage_pml<S, TX, TY, 0, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
-        case 1: k_stage_pml<S, TX, TY, 1, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
-        case 2: k_stage_pml<S, TX, TY, 2, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
-        default: k_stage_pml<S, TX, TY, 3, AXIS><<<grid, TX * TY, 0, s>>>(q, t); break;
+        case 0: k_stage_pml<S, TX, TY, 0, MASK><<<grid, TX * TY, 0, s>>>(q, t); break;
+        case 1: k_stage_pml<S, TX, TY, 1, MASK><<<grid, TX * TY, 0, s>>>(q, t); break;
+        case 2: k_stage_pml<S, TX, TY, 2, MASK><<<grid, TX * TY, 0, s>>>(q, t); break;
+        default: k_stage_pml<S, TX, TY, 3, MASK><<<grid, TX * TY, 0, s>>>(q, t); break;
     }
 }
 
 template <class S>
-static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid) {
+static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid, const StageFork* fk) {
     StageParams q = p;
     stage_boxes(q, use_fast);
+    // boundary kernels on the side stream, after everything enqueued on s so far
+    const bool forked = fk && fk->side && use_fast != 2;
+    if (forked) {
+        if (hipEventRecord(fk->fork, s) != hipSuccess || hipStreamWaitEvent(fk->side, fk->fork, 0) != hipSuccess)
+            return -3;
+    }
+    const hipStream_t sb = forked ? fk->side : s;
     if (q.bulk_on) {
         const int cfg = (p.bulk_cfg >= 0 && p.bulk_cfg < kNumBulkCfg) ? p.bulk_cfg : 0;
         const int TX = kBulkCfg[cfg].tx, TY = kBulkCfg[cfg].ty;
@@ -175,28 +193,38 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
     }
     if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;
     if (use_fast != 2) {
-        launch_pml<S, 16, 16, 0>(q, s);
-        launch_pml<S, 64, 4, 1>(q, s);
-        launch_pml<S, 64, 4, 2>(q, s);
+        const PmlTiles t = pml_tiles(q);
+        launch_pml<S, 1>(q, t, sb);
+        launch_pml<S, 2>(q, t, sb);
+        launch_pml<S, 4>(q, t, sb);
+        launch_pml<S, 3>(q, t, sb);
+        launch_pml<S, 5>(q, t, sb);
+        launch_pml<S, 6>(q, t, sb);
+        launch_pml<S, 7>(q, t, sb);
         if (hipGetLastError() != hipSuccess) return -3;
     }
     if (q.fs_on && use_fast != 2) {
         const int gx = (q.bi1 - q.bi0 + kFsTx - 1) / kFsTx, gy = (q.bj1 - q.bj0 + kFsTy - 1) / kFsTy;
-        k_stage_fs<S, kFsTx, kFsTy><<<(unsigned)(gx * gy), kFsTx * kFsTy, 0, s>>>(q, gx, gy);
+        k_stage_fs<S, kFsTx, kFsTy><<<(unsigned)(gx * gy), kFsTx * kFsTy, 0, sb>>>(q, gx, gy);
     }
     const long long n = q.boxStart[q.nbox];
     if (n > 0 && use_fast != 2) {
         const int bs = 256;
-        k_stage_general<S><<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(q);
+        k_stage_general<S><<<(unsigned)((n + bs - 1) / bs), bs, 0, sb>>>(q);
     }
-    return hipGetLastError() == hipSuccess ? 0 : -3;
+    if (hipGetLastError() != hipSuccess) return -3;
+    if (forked) {
+        if (hipEventRecord(fk->join, fk->side) != hipSuccess || hipStreamWaitEvent(s, fk->join, 0) != hipSuccess)
+            return -3;
+    }
+    return 0;
 }
 
-int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid) {
+int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid, const StageFork* fk) {
     switch (prec) {
-        case P_F32: return launch_stage_t<float>(p, use_fast, s, mid);
-        case P_F16: return launch_stage_t<_Float16>(p, use_fast, s, mid);
-        case P_BF16: return launch_stage_t<__bf16>(p, use_fast, s, mid);
+        case P_F32: return launch_stage_t<float>(p, use_fast, s, mid, fk);
+        case P_F16: return launch_stage_t<_Float16>(p, use_fast, s, mid, fk);
+        case P_BF16: return launch_stage_t<__bf16>(p, use_fast, s, mid, fk);
     }
     return -1;
 }

@@ -158,16 +158,18 @@ __device__ __forceinline__ void store9(Rsrc r, unsigned col, unsigned col1, cons
 // [ci0,ci1) x [cj0,cj1), planes [kb, ke). Every thread of the workgroup calls it with the same
 // arguments.
 //
-// AXIS < 0: the bulk box, where every PML factor is 1. AXIS = 0, 1, 2: a single-slab PML box of
-// slab `sid` whose normal is that axis (no other slab and no free-surface layer in the box):
-// the derivatives along AXIS are scaled by 1/beta of the point, and the slab's auxiliary
-// variables get their update (pml_deriv.cpp:176-237: -beta*Aux*DT into the increment, the
-// single-direction right-hand side times d minus (alpha+d)*Aux into the auxiliary increment;
-// pml_rk.cpp:14-70: RK4 on the auxiliaries).
-template <class S, int TX, int TY, int IRK, int AXIS = -1>
+// MASK = 0: the bulk box, where every PML factor is 1. MASK != 0: a PML box inside the slabs
+// whose normals are the set bits of MASK (bit 0 x, 1 y, 2 z; `slabs` holds the slab index of each
+// axis in 4 bits) and outside the free-surface layers: the derivatives along each such axis are
+// scaled by 1/beta of the point, and each slab's auxiliary variables get their update
+// (pml_deriv.cpp:176-237: -beta*Aux*DT into the increment, the single-direction right-hand side
+// times d minus (alpha+d)*Aux into the auxiliary increment; pml_rk.cpp:14-70: RK4 on the
+// auxiliaries), slab after slab in axis order as the reference's x, y, z launches.
+template <class S, int TX, int TY, int IRK, int MASK = 0>
 __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int ci0, int ci1, int cj0,
-                                          int cj1, int sid, int bx, int by, int kb, int ke,
+                                          int cj1, int slabs, int bx, int by, int kb, int ke,
                                           float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+    constexpr bool PML = MASK != 0;
     static_assert(kBlk % TX == 0, "a tile row must lie inside one layout block");
     using PR = Pair<S>;
     using P2 = typename PR::type;
@@ -186,7 +188,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const Layout L = p.lay;
     // x tiles of a PML box start on a TX-aligned run of a layout block (pml_tile_x0), so a row of
     // a tile never straddles two blocks; the bulk box starts a block
-    const int i0 = (AXIS >= 0 ? pml_tile_x0(L, ci0, TX) : ci0) + bx * TX, j0 = cj0 + by * TY;
+    const int i0 = (PML ? pml_tile_x0(L, ci0, TX) : ci0) + bx * TX, j0 = cj0 + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
     const bool has = i >= 0 && i < L.nx_ && j < L.ny_;       // a grid point (possibly outside the box)
@@ -280,44 +282,35 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         for (int c = 0; c < 4; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
     };
     if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
-    // PML slab of the box: per-lane profile values on the x/y axes, aux positions
-    // pos = base + kk*kstride with kk the plane index inside the slab's k range
-    [[maybe_unused]] float pBeta = 1.f, pD = 0.f, pAD = 0.f;
-    [[maybe_unused]] long long pBase = 0, pKs = 0;
-    [[maybe_unused]] int pK0 = 0, pKn = 1;
-    [[maybe_unused]] float *xA = nullptr, *xM = nullptr, *xT = nullptr;
-    [[maybe_unused]] long long xN = 0;
-    [[maybe_unused]] int xn4 = 0;
-    if constexpr (AXIS >= 0) {
-        const Slab& X = p.slab[sid];
-        const int np = p.nPML;
-        if constexpr (AXIS == 0) {
-            pBase = inside ? (long long)(i - X.lo) + (long long)(j - HALO) * np : 0;
-            pKs = (long long)np * p.ny;
-            pK0 = HALO;
-            pKn = p.nz;
-        } else if constexpr (AXIS == 1) {
-            pBase = inside ? (long long)(i - HALO) + (long long)(j - X.lo) * p.nx : 0;
-            pKs = (long long)p.nx * np;
-            pK0 = HALO;
-            pKn = p.nz;
-        } else {
-            pBase = inside ? (long long)(i - HALO) + (long long)(j - HALO) * p.nx : 0;
-            pKs = (long long)p.nx * p.ny;
-            pK0 = X.lo;
-            pKn = np;
+    // PML slabs of the box, per normal axis a: the x/y profile values of the lane, and the
+    // auxiliary positions pos = base + kk*kstride with kk the plane index inside the slab's k range
+    [[maybe_unused]] float pBeta[3] = {1.f, 1.f, 1.f}, pD[3] = {0.f, 0.f, 0.f}, pAD[3] = {0.f, 0.f, 0.f};
+    [[maybe_unused]] long long pBase[3] = {0, 0, 0}, pKs[3] = {0, 0, 0}, xN[3] = {0, 0, 0};
+    [[maybe_unused]] int pK0[3] = {0, 0, 0}, pKn[3] = {1, 1, 1}, xn4[3] = {0, 0, 0};
+    [[maybe_unused]] float* xA[3] = {nullptr, nullptr, nullptr};
+    [[maybe_unused]] float* xM[3] = {nullptr, nullptr, nullptr};
+    [[maybe_unused]] float* xT[3] = {nullptr, nullptr, nullptr};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!((MASK >> a) & 1)) continue;
+        const Slab& X = p.slab[(slabs >> (4 * a)) & 15];
+        const int ii = a == 0 ? i - X.lo : i - HALO;
+        const int jj = a == 1 ? j - X.lo : j - HALO;
+        pBase[a] = inside ? (long long)ii + (long long)jj * X.n0 : 0;
+        pKs[a] = (long long)X.n0 * X.n1;
+        pK0[a] = a == 2 ? X.lo : HALO;
+        pKn[a] = a == 2 ? p.nPML : p.nz;
+        if (a < 2) {
+            const int c = a == 0 ? (inside ? i : HALO) : (inside ? j : HALO);
+            pBeta[a] = p.pb[a][c];
+            pD[a] = p.pd[a][c];
+            pAD[a] = pD[a] + p.pa[a][c];
         }
-        if constexpr (AXIS < 2) {
-            const int c = AXIS == 0 ? (inside ? i : HALO) : (inside ? j : HALO);
-            pBeta = p.pb[AXIS][c];
-            pD = p.pd[AXIS][c];
-            pAD = pD + p.pa[AXIS][c];
-        }
-        xA = X.a;
-        xM = X.m;
-        xT = X.t;
-        xN = X.n;
-        xn4 = (int)(X.n * 4);  // bytes per auxiliary variable
+        xA[a] = X.a;
+        xM[a] = X.m;
+        xT[a] = X.t;
+        xN[a] = X.n;
+        xn4[a] = (int)(X.n * 4);  // bytes per auxiliary variable
     }
 
     auto step = [&](auto Rc, int it) {
@@ -357,30 +350,32 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         W9<S> ra, rt;
         if constexpr (IRK == 1 || IRK == 2) ra.template load<PB, CPS>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
         if constexpr (IRK >= 2) rt.template load<PB, CPS>(rT, col, col1);
-        // auxiliaries of this plane (lanes outside the box read element 0)
-        [[maybe_unused]] float xa[9], xm[9], xt[9];
-        [[maybe_unused]] unsigned xoff = 0;
-        [[maybe_unused]] float zBeta = 1.f, zD = 0.f, zAD = 0.f;
-        [[maybe_unused]] const Rsrc rXa = make_rsrc(xA, 9 * xN);
-        [[maybe_unused]] const Rsrc rXm = make_rsrc(xM, 9 * xN);
-        [[maybe_unused]] const Rsrc rXt = make_rsrc(xT, 9 * xN);
-        if constexpr (AXIS >= 0) {
-            const int kk = min(max(kc - pK0, 0), pKn - 1);
-            xoff = inside ? (unsigned)((pBase + (long long)kk * pKs) * 4) : 0u;
+        // auxiliaries of this plane (lanes outside the box read element 0); the z profile
+        // values belong to the plane
+        [[maybe_unused]] float xa[3][9], xm[3][9], xt[3][9];
+        [[maybe_unused]] unsigned xoff[3] = {0u, 0u, 0u};
 #pragma unroll
-            for (int v = 0; v < 9; ++v) xa[v] = Pair<float>::template ld1<CPX>(rXa, xoff, v * xn4);
+        for (int a = 0; a < 3; ++a) {
+            if (!((MASK >> a) & 1)) continue;
+            const int kk = min(max(kc - pK0[a], 0), pKn[a] - 1);
+            xoff[a] = inside ? (unsigned)((pBase[a] + (long long)kk * pKs[a]) * 4) : 0u;
+            const Rsrc rXa = make_rsrc(xA[a], 9 * xN[a]);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) xa[a][v] = Pair<float>::template ld1<CPX>(rXa, xoff[a], v * xn4[a]);
             if constexpr (IRK == 1 || IRK == 2) {
+                const Rsrc rXm = make_rsrc(xM[a], 9 * xN[a]);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xm[v] = Pair<float>::template ld1<CPX>(rXm, xoff, v * xn4);
+                for (int v = 0; v < 9; ++v) xm[a][v] = Pair<float>::template ld1<CPX>(rXm, xoff[a], v * xn4[a]);
             }
             if constexpr (IRK >= 2) {
+                const Rsrc rXt = make_rsrc(xT[a], 9 * xN[a]);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xt[v] = Pair<float>::template ld1<CPX>(rXt, xoff, v * xn4);
+                for (int v = 0; v < 9; ++v) xt[a][v] = Pair<float>::template ld1<CPX>(rXt, xoff[a], v * xn4[a]);
             }
-            if constexpr (AXIS == 2) {
-                zBeta = p.pb[2][kc];
-                zD = p.pd[2][kc];
-                zAD = zD + p.pa[2][kc];
+            if (a == 2) {
+                pBeta[2] = p.pb[2][kc];
+                pD[2] = p.pd[2][kc];
+                pAD[2] = pD[2] + p.pa[2][kc];
             }
         }
         if (inside && active) {
@@ -398,7 +393,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             // so only the three derivatives of one variable are live at a time.
             float G[3][3];
             float dv[3] = {0.f, 0.f, 0.f};
-            [[maybe_unused]] float draw[9];  // raw derivative along the slab normal
+            [[maybe_unused]] float draw[3][9];  // raw derivatives along the slab normals
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const float* Lv = Lb + v * VS;
@@ -406,9 +401,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 float d0 = cx0 * Lv[ox0] + cx1 * c + cx2 * Lv[ox2] + cx3 * Lv[ox3] + cx4 * Lv[ox4];
                 float d1 = cy0 * Lv[oy0] + cy1 * c + cy2 * Lv[oy2] + cy3 * Lv[oy3] + cy4 * Lv[oy4];
                 float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
-                if constexpr (AXIS == 0) { draw[v] = d0; d0 *= pBeta; }
-                if constexpr (AXIS == 1) { draw[v] = d1; d1 *= pBeta; }
-                if constexpr (AXIS == 2) { draw[v] = d2; d2 *= zBeta; }
+                if constexpr ((MASK & 1) != 0) { draw[0][v] = d0; d0 *= pBeta[0]; }
+                if constexpr ((MASK & 2) != 0) { draw[1][v] = d1; d1 *= pBeta[1]; }
+                if constexpr ((MASK & 4) != 0) { draw[2][v] = d2; d2 *= pBeta[2]; }
                 if (v < 3) {
 #pragma unroll
                     for (int cc = 0; cc < 3; ++cc) G[v][cc] = phys(cc, d0, d1, d2);
@@ -433,21 +428,20 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             h[6] = mu * (G[0][1] + G[1][0]);
             h[7] = mu * (G[0][2] + G[2][0]);
             h[8] = mu * (G[1][2] + G[2][1]);
-            if constexpr (AXIS >= 0) {
-                const float beta = AXIS == 2 ? zBeta : pBeta;
-                const float dd = AXIS == 2 ? zD : pD;
-                const float ad = AXIS == 2 ? zAD : pAD;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (!((MASK >> a) & 1)) continue;
                 // -beta*Aux*DT into the increment, then the single-direction right-hand side
                 // (with d, no beta) of the slab normal
 #pragma unroll
-                for (int v = 0; v < 9; ++v) h[v] += -beta * xa[v] * DT;
+                for (int v = 0; v < 9; ++v) h[v] += -pBeta[a] * xa[a][v] * DT;
                 // metric row of the slab normal (zero entries as above)
-                const float mx = AXIS == 0 ? xix : AXIS == 2 ? ztx : 0.f;
-                const float my = AXIS == 1 ? ety : AXIS == 2 ? zty : 0.f;
-                const float mz = AXIS == 2 ? ztz : 0.f;
+                const float mx = a == 0 ? xix : a == 2 ? ztx : 0.f;
+                const float my = a == 1 ? ety : a == 2 ? zty : 0.f;
+                const float mz = a == 2 ? ztz : 0.f;
                 float b[9];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) b[v] = draw[v] * dd;
+                for (int v = 0; v < 9; ++v) b[v] = draw[a][v] * pD[a];
                 float ha[9];
                 ha[0] = (mx * b[3] + my * b[6] + mz * b[7]) * bu;
                 ha[1] = (mx * b[6] + my * b[4] + mz * b[8]) * bu;
@@ -459,22 +453,27 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 ha[6] = (my * b[0] + mx * b[1]) * mu;
                 ha[7] = (mz * b[0] + mx * b[2]) * mu;
                 ha[8] = (mz * b[1] + my * b[2]) * mu;
+                const Rsrc rXa = make_rsrc(xA[a], 9 * xN[a]);
+                const Rsrc rXm = make_rsrc(xM[a], 9 * xN[a]);
+                const Rsrc rXt = make_rsrc(xT[a], 9 * xN[a]);
+                const unsigned xo = xoff[a];
+                const int x4 = xn4[a];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
-                    const float hav = ha[v] - ad * xa[v] * DT;
+                    const float hav = ha[v] - pAD[a] * xa[a][v] * DT;
                     // as for the wavefield, the accumulator after stage 0 is rebuilt in stage 1
                     if constexpr (IRK == 0) {
-                        Pair<float>::template st1<CPX>(rXm, xoff, v * xn4, xa[v]);
-                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xa[v] + alpha2 * hav);
+                        Pair<float>::template st1<CPX>(rXm, xo, v * x4, xa[a][v]);
+                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xa[a][v] + alpha2 * hav);
                     } else if constexpr (IRK == 1) {
-                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4,
-                                                       (xm[v] + (xa[v] - xm[v]) * kT0) + beta2 * hav);
-                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xm[v] + alpha3 * hav);
+                        Pair<float>::template st1<CPX>(rXt, xo, v * x4,
+                                                       (xm[a][v] + (xa[a][v] - xm[a][v]) * kT0) + beta2 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + alpha3 * hav);
                     } else if constexpr (IRK == 2) {
-                        Pair<float>::template st1<CPX>(rXt, xoff, v * xn4, xt[v] + beta3 * hav);
-                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xm[v] + hav);
+                        Pair<float>::template st1<CPX>(rXt, xo, v * x4, xt[a][v] + beta3 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + hav);
                     } else {
-                        Pair<float>::template st1<CPX>(rXa, xoff, v * xn4, xt[v] + beta4 * hav);
+                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xt[a][v] + beta4 * hav);
                     }
                 }
             }
@@ -541,27 +540,49 @@ k_stage_bulk(StageParams p, int kc, int gx, int gy) {
     bulk_tile<S, TX, TY, IRK>(p, p.FB1, p.FB2, p.FB3, p.bi0, p.bi1, p.bj0, p.bj1, 0, bx, by, kb, ke, lds.v);
 }
 
-// Tiling of the single-slab PML boxes of one normal axis (one launch per axis): box e of the
-// list is p.pbox[box[e]], cut into gx*gy column tiles and nch chunks of kc planes; tile indices
-// are consecutive per box (start[e]).
-struct PmlTiling {
+// Tiling of the PML boxes: entry e is box p.pbox[box[e]], cut into gx*gy column tiles and nch
+// chunks of kc planes; entries are grouped by box kind (pml_mask_at order, group g holds entries
+// first[g] .. first[g+1]-1) and tile indices run consecutively over all entries (start[e]). One
+// launch per kind covers the tiles start[first[g]] .. start[first[g+1]]-1.
+struct PmlTiles {
     int n;
-    int box[2];
-    int gx[2], gy[2], nch[2], kc[2];
-    int start[3];
+    int first[kNumPmlMasks + 1];
+    int box[kMaxPBoxes];
+    int gx[kMaxPBoxes], gy[kMaxPBoxes], nch[kMaxPBoxes], kc[kMaxPBoxes];
+    int start[kMaxPBoxes + 1];
 };
 
-template <class S, int TX, int TY, int IRK, int AXIS>
-__global__ void __launch_bounds__(TX* TY) k_stage_pml(StageParams p, PmlTiling tl) {
-    __shared__ BulkLds<TX, TY> lds;
-    const int t = (int)xcd_order(blockIdx.x, (long long)gridDim.x);
-    const int e = (tl.n > 1 && t >= tl.start[1]) ? 1 : 0;
-    const int r = t - tl.start[e];
+// Tile shape of a PML box kind: 16x16 tiles aligned inside a layout block for boxes in an x slab
+// (12 of 16 columns used instead of 12 of 64), 64x4 otherwise.
+__host__ __device__ __forceinline__ constexpr int pml_tx(int mask) { return (mask & 1) ? 16 : 64; }
+__host__ __device__ __forceinline__ constexpr int pml_ty(int mask) { return (mask & 1) ? 16 : 4; }
+
+// Entry of tile t (a global tile index) among entries e0 .. e1-1.
+__device__ __forceinline__ int pml_entry(const PmlTiles& tl, int e0, int e1, int t) {
+    int e = e0;
+    while (e + 1 < e1 && t >= tl.start[e + 1]) ++e;
+    return e;
+}
+
+// Tile (bx, by, chunk) of entry e.
+template <class S, int TX, int TY, int IRK, int MASK>
+__device__ __forceinline__ void pml_tile(const StageParams& p, const PmlTiles& tl, int e, int r,
+                                         float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
     const int bx = r % tl.gx[e], by = (r / tl.gx[e]) % tl.gy[e], bz = r / (tl.gx[e] * tl.gy[e]);
     const int* B = p.pbox[tl.box[e]];
     const int kb = B[4] + bz * tl.kc[e];
     const int ke = min(kb + tl.kc[e], B[5]);
-    bulk_tile<S, TX, TY, IRK, AXIS>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[6], bx, by, kb, ke, lds.v);
+    bulk_tile<S, TX, TY, IRK, MASK>(p, p.FB1, p.FB2, p.FB3, B[0], B[1], B[2], B[3], B[7], bx, by, kb, ke, lds);
+}
+
+template <class S, int TX, int TY, int IRK, int MASK>
+__global__ void __launch_bounds__(TX* TY) k_stage_pml(StageParams p, PmlTiles tl) {
+    __shared__ BulkLds<TX, TY> lds;
+    constexpr int g = pml_group_of(MASK);
+    const int e0 = tl.first[g], e1 = tl.first[g + 1];
+    const int t = tl.start[e0] + (int)xcd_order(blockIdx.x, (long long)gridDim.x);
+    const int e = pml_entry(tl, e0, e1, t);
+    pml_tile<S, TX, TY, IRK, MASK>(p, tl, e, t - tl.start[e], lds.v);
 }
 
 }  // namespace cg

@@ -153,8 +153,8 @@ int cgfdm_kernel_timing(cgfdm_domain* h, int reset, double* out, long long* coun
 /* Enables/disables HIP-event timing of the stage kernels (off by default). */
 int cgfdm_enable_timing(cgfdm_domain* h, int on);
 
-/* How a stage is split: out[0] points of the bulk box, out[1] points of the general boxes and
- * of the free-surface box,
+/* How a stage is split: out[0] points of the bulk box, out[1] the other interior points (PML,
+ * free-surface and general boxes),
  * out[2] bulk tile shape (0: 64x8, 1: 64x4, -1: not tuned yet), out[3] z-chunk (0: automatic). */
 int cgfdm_partition(const cgfdm_domain* h, long long* out);
 
@@ -162,9 +162,11 @@ int cgfdm_partition(const cgfdm_domain* h, long long* out);
  * writes up to max_boxes records of 8 ints (i0, i1, j0, j1, k0, k1 in padded indices, kind,
  * axis) and returns the number of boxes (or a negative status). kind 0: the bulk box (tile
  * kernel), 1: a single-slab PML box whose normal is `axis` (tile kernel with the auxiliary
- * update), 2: a general box (point kernel: slab edges and corners, the free-surface layers of
- * the PML columns), 3: the free-surface layers above the bulk columns (free-surface tile
- * kernel). The boxes are disjoint and cover the interior. No counterpart in the reference, which loops over
+ * update), 2: a general box (point kernel: the free-surface layers of the PML columns), 3: the
+ * free-surface layers above the bulk columns (free-surface tile kernel), 4: a PML box where two
+ * or three slabs meet (an edge or a corner; `axis` is the mask of their normals, bit 0 x, 1 y,
+ * 2 z; tile kernel with every slab's auxiliary update). The boxes are disjoint and cover the
+ * interior. No counterpart in the reference, which loops over
  * the whole grid in every kernel. */
 int cgfdm_decompose(const cgfdm_desc* desc, int* boxes, int max_boxes);
 

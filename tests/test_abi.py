@@ -89,9 +89,10 @@ def test_no_fallback_when_library_missing(gpu, monkeypatch):
                                                (1, 0, 6, (20, 18, 12)), (0, 0, 5, (16, 16, 16)),
                                                (1, 1, 8, (18, 40, 30)), (1, 1, 4, (9, 9, 10))])
 def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
-    """Bulk box, single-slab PML boxes, the free-surface box and general boxes tile the interior
-    exactly once; a PML box lies inside exactly one slab (its normal axis) and outside the
-    free-surface layers; the free-surface box is the top three layers of the bulk columns."""
+    """Bulk box, PML face/edge/corner boxes, the free-surface box and general boxes tile the
+    interior exactly once; a face box lies inside exactly one slab (its normal axis), an edge or
+    corner box inside exactly the slabs of its mask, all outside the free-surface layers; the
+    free-surface box is the top three layers of the bulk columns."""
     import numpy as np
     NX, NY, NZ = shape
     d = gpu.make_desc(NX, NY, NZ, nPML=npml, free_surface=fs, pml=pml)
@@ -112,6 +113,17 @@ def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
                     assert b0 >= 3 + npml and b1 <= top
             if fs:
                 assert k1 <= 3 + NZ - 3
+        if kind == 4:  # edge or corner: inside exactly the slabs of the normals in the mask
+            assert axis in (3, 5, 6, 7)
+            ranges = [(i0, i1, NX), (j0, j1, NY), (k0, k1, NZ)]
+            for a, (b0, b1, n) in enumerate(ranges):
+                in_low = b1 <= 3 + npml
+                in_high = b0 >= 3 + n - npml and not (a == 2 and fs)
+                top = 3 + n - (3 if (a == 2 and fs) else npml)
+                if (axis >> a) & 1:
+                    assert in_low or in_high
+                else:
+                    assert b0 >= 3 + npml and b1 <= top
         if kind == 3:  # free-surface layers above the bulk columns
             assert fs and (k0, k1) == (3 + NZ - 3, 3 + NZ)
             np_ = npml if pml else 0
@@ -123,3 +135,5 @@ def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
     if pml and min(NX, NY, NZ) > 2 * npml + 3:
         assert kinds.count(0) == 1 and kinds.count(1) == (5 if fs else 6)
         assert kinds.count(3) == (1 if fs else 0)
+        assert kinds.count(4) == (12 if fs else 20)  # edges and corners below the surface layers
+        assert kinds.count(2) == (4 if fs else 0)    # the surface layers of the PML columns
