@@ -192,11 +192,13 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True)
     return np.array(lat), last - first, wall, stats
 
 
-def serving(n, seed, max_domains, steps_per_launch, use_graph=True):
+def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batched", "one_at_a_time")):
     reqs = wl.serving_stream(n, seed)
     lups = sum(r["grid"][0] * r["grid"][1] * r["grid"][2] * r["NT"] for _, r in reqs)
     out = {}
     for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
+        if name not in modes:
+            continue
         lat, span, wall, st = serving_run(reqs, md, steps_per_launch, use_graph=use_graph)
         out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
@@ -210,7 +212,8 @@ def serving(n, seed, max_domains, steps_per_launch, use_graph=True):
                      f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
                      f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface "
                      f"stations")
-    out["speedup"] = round(out["batched"]["solves_per_s"] / out["one_at_a_time"]["solves_per_s"], 3)
+    if "batched" in out and "one_at_a_time" in out:
+        out["speedup"] = round(out["batched"]["solves_per_s"] / out["one_at_a_time"]["solves_per_s"], 3)
     return out
 
 
@@ -247,9 +250,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving stream to run (0: skip; 512: the whole stream)")
-    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-steps-per-launch", type=int, default=8)
     ap.add_argument("--serve-graph", type=int, default=1, help="1: replay captured HIP graphs, 0: launch directly")
+    ap.add_argument("--serve-modes", default="batched,one_at_a_time", help="comma list of batched, one_at_a_time")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -272,7 +276,8 @@ def main():
         for prec in precs:
             res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
         del model
-    serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph)) \
+    serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph),
+                    tuple(args.serve_modes.split(","))) \
         if args.serve_requests > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
@@ -316,7 +321,7 @@ def main():
         out[f"{prec}_store"] = {k: (round(x, 4) if isinstance(x, float) else x) for k, x in v.items()}
     if "fp16" in res and "fp32" in res:
         out["fp16_over_fp32"] = round(res["fp16"]["glups"] / res["fp32"]["glups"], 3)
-    if serve:
+    if serve and "batched" in serve:
         out["latency_p50_ms"] = round(serve["batched"]["p50_s"] * 1e3, 1)
         out["latency_p99_ms"] = round(serve["batched"]["p99_s"] * 1e3, 1)
         out["serving"] = serve

@@ -15,6 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcgfdm.so")
 CLI = os.path.join(HERE, "cgfdm_run")
 SOURCES = ["setup.hip", "stage.hip", "misc.hip", "api.hip", "batch.hip", "server.hip"]
+HOST_SOURCES = ["finite_fault.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 on the device (the host pass only warns about it).
@@ -42,6 +43,14 @@ def build(jobs=4, verbose=False):
         if verbose and msgs.strip():
             print(msgs)
     objs = [o for o, _ in results]
+    # host-only sources (no device code): the finite-fault reader
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        r = subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}{r.stderr}")
+        objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -93,6 +93,9 @@ def lib():
         L.cgfdm_partition.argtypes = [vp, vp]
         L.cgfdm_decompose.argtypes = [vp, vp, i]
         L.cgfdm_set_model_layered.argtypes = [vp, vp, i, vp, vp, vp, vp]
+        L.cgfdm_finite_fault_map.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, ll,
+                                             P(ll), P(i), P(f)]
+        L.cgfdm_aeqd_forward.argtypes = [ctypes.c_double, ctypes.c_double, ll, vp, vp, vp, vp]
         L.cgfdm_server_create.argtypes = [vp, P(vp)]
         L.cgfdm_server_submit.argtypes = [vp, vp, P(ll)]
         L.cgfdm_server_wait.argtypes = [vp, vp, i, i]
@@ -245,6 +248,31 @@ def decompose(desc):
     if n < 0:
         _chk(n)
     return [tuple(int(x) for x in r) for r in out[:n]]
+
+
+def finite_fault_map(path, desc, terrain, lon0, lat0):
+    """The reference's finite-fault file mapped onto the grid of `desc` (host only):
+    (xyz int32 [n, 3] global interior indices, rate float32 [nt, n, 6], dt) as
+    set_source_table takes them."""
+    L = lib()
+    n, nt, dt = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_float()
+    tp = None if terrain is None else np.ascontiguousarray(terrain, np.float64)
+    _chk(L.cgfdm_finite_fault_map(str(path).encode(), ctypes.byref(desc), _p(tp), lon0, lat0, None, None, 0,
+                                  ctypes.byref(n), ctypes.byref(nt), ctypes.byref(dt)))
+    xyz = np.zeros((max(n.value, 1), 3), np.int32)
+    rate = np.zeros((nt.value, max(n.value, 1), 6), np.float32)
+    _chk(L.cgfdm_finite_fault_map(str(path).encode(), ctypes.byref(desc), _p(tp), lon0, lat0, _p(xyz), _p(rate),
+                                  n.value, ctypes.byref(n), ctypes.byref(nt), ctypes.byref(dt)))
+    return xyz[:n.value], rate[:, :n.value], dt.value
+
+
+def aeqd_forward(lon0, lat0, lon, lat):
+    """Azimuthal equidistant projection on WGS84 centred on (lon0, lat0): degrees -> metres."""
+    lon = np.ascontiguousarray(lon, np.float64)
+    lat = np.ascontiguousarray(lat, np.float64)
+    x, y = np.zeros_like(lon), np.zeros_like(lat)
+    _chk(lib().cgfdm_aeqd_forward(lon0, lat0, lon.size, _p(lon), _p(lat), _p(x), _p(y)))
+    return x, y
 
 
 def estimate_bytes(desc):

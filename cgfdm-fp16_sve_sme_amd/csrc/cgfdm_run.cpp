@@ -180,6 +180,8 @@ struct Params {
     std::string LayeredFileName, OUT = ".";
     double Depth = 0;  // km
     int sourceX = 0, sourceY = 0, sourceZ = 0;
+    std::string sourceFile, sourceDir = ".";
+    double centerLongitude = 0, centerLatitude = 0;
 };
 
 // getParams (getParams.cpp:55-247)
@@ -211,6 +213,10 @@ int load_params(const std::string& path, Params& P, std::string& err) {
     P.OUT = sval(root, "OUT", ".");
     P.Depth = num(root, "Depth(km)", 0);
     P.sourceX = inum(root, "sourceX", 0); P.sourceY = inum(root, "sourceY", 0); P.sourceZ = inum(root, "sourceZ", 0);
+    P.sourceFile = sval(root, "sourceFile", "");
+    P.sourceDir = sval(root, "sourceDir", ".");
+    P.centerLongitude = num(root, "centerLongitude", 0);
+    P.centerLatitude = num(root, "centerLatitude", 0);
     if (!P.useTerrain) P.Depth = P.DH * P.NZ * 1e-3;
     if (P.IT_SKIP <= 0) P.IT_SKIP = 1;
     return 0;
@@ -300,8 +306,8 @@ int main(int argc, char** argv) {
     if (int code = load_params(paramsPath, P, err)) { std::fprintf(stderr, "%s\n", err.c_str()); return code == 1001 ? 101 : code; }
     if (P.useTerrain) { std::fprintf(stderr, "useTerrain (SRTM + PROJ) is not supported; use gauss_hill or a flat grid\n"); return 1; }
     if (P.useMedium && (P.ShenModel || P.Crust_1Medel)) { std::fprintf(stderr, "only homogeneous and layered media are supported\n"); return 1; }
-    if (P.useMultiSource) { std::fprintf(stderr, "useMultiSource needs PROJ-projected source files; not supported by this driver\n"); return 1; }
-    if (!P.useSingleSource) { std::fprintf(stderr, "No source configured (modelChecking)\n"); return 180; }
+    if (!P.useSingleSource && !P.useMultiSource) { std::fprintf(stderr, "No source configured (modelChecking)\n"); return 180; }
+    if (P.useSingleSource && P.useMultiSource) { std::fprintf(stderr, "useSingleSource together with useMultiSource is not supported\n"); return 1; }
 
     const int NX = P.NX, NY = P.NY, NZ = P.NZ;
     const int nx_ = NX + 6, ny_ = NY + 6;
@@ -435,8 +441,31 @@ int main(int argc, char** argv) {
     rc = cgfdm_set_model(h, P.gauss_hill ? terrain.data() : nullptr, vs.data(), vp.data(), rho.data());
     if (rc == CGFDM_ECFL) { std::fprintf(stderr, "The parameters can't afford the CFL condition! %s\n", cgfdm_last_error()); return 110; }
     if (rc) { std::fprintf(stderr, "set_model: %s\n", cgfdm_last_error()); return 1; }
-    rc = cgfdm_set_source_point(h, P.sourceX, P.sourceY, P.sourceZ, P.rickerfc);
-    if (rc) { std::fprintf(stderr, "source: %s\n", cgfdm_last_error()); return 1; }
+    if (P.useMultiSource) {
+        // init_MultiSource (multiSource.cpp:824-922): the finite-fault file mapped onto the grid
+        const std::string path = P.sourceDir + "/" + P.sourceFile;
+        const double* tp = P.gauss_hill ? terrain.data() : nullptr;
+        long long n = 0;
+        int snt = 0;
+        float sdt = 0.f;
+        rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, nullptr, nullptr, 0, &n,
+                                    &snt, &sdt);
+        std::vector<int> sxyz((size_t)std::max<long long>(n, 1) * 3);
+        std::vector<float> srate((size_t)std::max<long long>(n, 1) * snt * 6);
+        if (!rc)
+            rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, sxyz.data(),
+                                        srate.data(), n, &n, &snt, &sdt);
+        if (rc) { std::fprintf(stderr, "source file: %s\n", cgfdm_last_error()); return 1; }
+        std::printf("finite fault %s: %lld grid nodes, nt = %d, dt = %g\n", path.c_str(), n, snt, sdt);
+        if (n > 0 && (rc = cgfdm_set_source_table(h, sxyz.data(), srate.data(), n, snt, sdt))) {
+            std::fprintf(stderr, "source: %s\n", cgfdm_last_error());
+            return 1;
+        }
+    }
+    if (P.useSingleSource) {
+        rc = cgfdm_set_source_point(h, P.sourceX, P.sourceY, P.sourceZ, P.rickerfc);
+        if (rc) { std::fprintf(stderr, "source: %s\n", cgfdm_last_error()); return 1; }
+    }
     if (nst && (rc = cgfdm_set_stations(h, st.data(), nst))) { std::fprintf(stderr, "stations: %s\n", cgfdm_last_error()); return 1; }
 
     auto vplane = [&](int var) {

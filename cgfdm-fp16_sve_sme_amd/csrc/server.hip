@@ -256,7 +256,9 @@ int cgfdm_server::setup_resident(Resident& R) {
         sd.fgx = (base.bi1 - base.bi0 + kFsTx - 1) / kFsTx;
         sd.fgy = (base.bj1 - base.bj0 + kFsTy - 1) / kFsTy;
     }
-    sd.pt = pml_tiles(base);
+    // the PML tiles of one domain are its share of a launch shared by up to 64 domains: ~96 per
+    // box kind, so chunks stay long (a domain's own 1024 would make them 5 planes)
+    sd.pt = pml_tiles(base, 96);
     sd.nbox = base.nbox;
     for (int x = 0; x < base.nbox; ++x)
         sd.boxChunks[x] = (base.boxStart[x + 1] - base.boxStart[x] + kGenChunk - 1) / kGenChunk;

@@ -113,11 +113,11 @@ void stage_boxes(StageParams& q, int use_fast) {
     }
 }
 
-// Tiles of every PML box, grouped by kind (pml_mask_at order); z chunks sized so each kind's
-// launch has about 1024 workgroups, with chunks of at least 5 planes (the tile loop's unit):
-// the small edge and corner boxes are cut into short chunks so their launches are not one long
-// z march.
-PmlTiles pml_tiles(const StageParams& q) {
+// Tiles of every PML box, grouped by kind (pml_mask_at order); z chunks sized so each kind has
+// about `want` tiles (a single domain: enough workgroups to fill the GPU; one of many batched
+// domains: its share), with chunks of at least 5 planes (the tile loop's unit): the small edge
+// and corner boxes are cut into short chunks so their launches are not one long z march.
+PmlTiles pml_tiles(const StageParams& q, int want) {
     PmlTiles t{};
     t.start[0] = 0;
     for (int g = 0; g < kNumPmlMasks; ++g) {
@@ -130,11 +130,10 @@ PmlTiles pml_tiles(const StageParams& q) {
             if (B[6] != mask) continue;
             const int e = t.n++;
             t.box[e] = b;
-            t.gx[e] = (B[1] - pml_tile_x0(q.lay, B[0], TX) + TX - 1) / TX;
+            t.gx[e] = (B[1] - pml_tile_x0(q.lay, B[0], B[1], TX) + TX - 1) / TX;
             t.gy[e] = (B[3] - B[2] + TY - 1) / TY;
             cols += t.gx[e] * t.gy[e];
         }
-        const int want = 1024;
         for (int e = t.first[g]; e < t.n; ++e) {
             const int* B = q.pbox[t.box[e]];
             const int nzb = B[5] - B[4];
@@ -193,7 +192,7 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
     }
     if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;
     if (use_fast != 2) {
-        const PmlTiles t = pml_tiles(q);
+        const PmlTiles t = pml_tiles(q, 1024);
         launch_pml<S, 1>(q, t, sb);
         launch_pml<S, 2>(q, t, sb);
         launch_pml<S, 4>(q, t, sb);

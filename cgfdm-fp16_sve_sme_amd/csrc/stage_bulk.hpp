@@ -29,9 +29,13 @@
 
 namespace cg {
 
-// First column of the x tiles of a box starting at column ci0: ci0 moved down to the start of
-// its TX-aligned run inside a layout block (TX divides the block size).
-__host__ __device__ __forceinline__ int pml_tile_x0(const Layout& L, int ci0, int TX) {
+// First column of the x tiles of a box with columns [ci0, ci1), chosen so that no tile row
+// straddles two layout blocks (TX divides the block size): a box no wider than a tile that fits
+// in one block from ci0 gets one tile starting at ci0; otherwise the tiles start on the TX-aligned
+// run of the block that holds ci0.
+__host__ __device__ __forceinline__ int pml_tile_x0(const Layout& L, int ci0, int ci1, int TX) {
+    const int off = (ci0 + L.lead) & (kBlk - 1);
+    if (ci1 - ci0 <= TX && off + TX <= kBlk) return ci0;
     return ci0 - ((ci0 + L.lead) & (TX - 1));
 }
 
@@ -186,9 +190,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const Layout L = p.lay;
-    // x tiles of a PML box start on a TX-aligned run of a layout block (pml_tile_x0), so a row of
-    // a tile never straddles two blocks; the bulk box starts a block
-    const int i0 = (PML ? pml_tile_x0(L, ci0, TX) : ci0) + bx * TX, j0 = cj0 + by * TY;
+    // x tiles of a PML box start where no tile row straddles two layout blocks (pml_tile_x0); the
+    // bulk box starts a block
+    const int i0 = (PML ? pml_tile_x0(L, ci0, ci1, TX) : ci0) + bx * TX, j0 = cj0 + by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int hx = FB1 > 0 ? 1 : 3, hy = FB2 > 0 ? 1 : 3;  // halo width on the low side
     const bool has = i >= 0 && i < L.nx_ && j < L.ny_;       // a grid point (possibly outside the box)

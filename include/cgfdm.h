@@ -92,6 +92,27 @@ long long cgfdm_estimate_bytes(const cgfdm_desc* desc);
 int cgfdm_set_model(cgfdm_domain* h, const double* terrain,
                     const float* vs, const float* vp, const float* rho);
 
+/* Finite-fault source file of the reference (useMultiSource, init_MultiSource,
+ * multiSource.cpp:824-922): int32 npts, int32 nt, float32 dt, then per point float32 lon, lat
+ * (degrees), z (m), area (m^2), strike, dip (degrees), rake[nt] (degrees), rate[nt] (slip rate).
+ * Maps it onto the grid of `desc` (and `terrain`, padded heights, when desc->use_terrain) as the
+ * reference does: azimuthal equidistant projection on WGS84 centred on (lon0, lat0) = the
+ * grid's (centerX, centerY) (projTrans, :109-173), depths relative to the shallowest point and
+ * the grid node of each point's cell (srcCoord2Index, :177-320), moment-rate tensors
+ * (solveMomentRate, :478-545), points sharing a node summed (dealDuplicateIndex, :551-607), nodes
+ * in increasing order (changeStorageOrder, :752-796). Fills xyz[3*n] (global interior indices)
+ * and rate[nt][n][6], the inputs of cgfdm_set_source_table, for the n nodes inside the grid
+ * (n <= cap); with xyz == rate == NULL only reports n, nt and dt. Host only. */
+int cgfdm_finite_fault_map(const char* path, const cgfdm_desc* desc, const double* terrain, double lon0,
+                           double lat0, int* xyz, float* rate, long long cap, long long* npts_out, int* nt_out,
+                           float* dt_out);
+
+/* The projection cgfdm_finite_fault_map uses: azimuthal equidistant on WGS84 centred on
+ * (lon0, lat0), degrees in, metres out (x east, y north); PROJ's "+proj=aeqd +ellps=WGS84" in
+ * the reference (multiSource.cpp:149-170). No counterpart call in the reference's own code. */
+int cgfdm_aeqd_forward(double lon0, double lat0, long long n, const double* lon, const double* lat, double* x,
+                       double* y);
+
 /* Single explosive Ricker source at a global interior index with peak frequency fc
  * (reference useSingleSource: locateSource/loadPointSource, singleSource.cpp:1-177,
  * propagate.cpp:145-146). M0 = 1e16 N*m as in the reference. */
