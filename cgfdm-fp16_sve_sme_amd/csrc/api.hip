@@ -238,7 +238,7 @@ struct TuneKey {
 };
 std::mutex g_tuneMu;
 std::map<TuneKey, std::pair<int, int>> g_tuned;
-const int kTuneCands[][2] = {{0, 0}, {0, 64}, {1, 0}, {1, 64}};
+const int kTuneCands[][2] = {{0, 0}, {0, 64}, {1, 0}, {1, 64}, {2, 0}, {2, 64}, {3, 0}, {3, 64}};
 
 int tune_bulk(cgfdm_domain* h, StageParams p) {
     const TuneKey key{h->prec, h->nx, h->ny, h->nz, h->d.nPML, h->d.free_surface};

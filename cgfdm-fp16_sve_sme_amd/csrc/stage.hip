@@ -9,8 +9,8 @@ namespace cg {
 
 // Bulk tile shapes (x by y columns per workgroup); the choice is a launch parameter.
 struct BulkCfg { int tx, ty; };
-static const BulkCfg kBulkCfg[] = {{64, 8}, {64, 4}};
-constexpr int kNumBulkCfg = 2;
+static const BulkCfg kBulkCfg[] = {{64, 8}, {64, 4}, {64, 16}, {128, 8}};
+constexpr int kNumBulkCfg = 4;
 
 template <class S, int TX, int TY>
 static void launch_bulk(const StageParams& q, unsigned grid, int kc, int gx, int gy, hipStream_t s) {
@@ -186,8 +186,12 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
         kc = std::max(1, std::min(kc, nzb));
         const int nch = (nzb + kc - 1) / kc;
         const unsigned grid = (unsigned)((long long)gx * gy * nch);
-        if (cfg == 0) launch_bulk<S, 64, 8>(q, grid, kc, gx, gy, s);
-        else launch_bulk<S, 64, 4>(q, grid, kc, gx, gy, s);
+        switch (cfg) {
+            case 0: launch_bulk<S, 64, 8>(q, grid, kc, gx, gy, s); break;
+            case 1: launch_bulk<S, 64, 4>(q, grid, kc, gx, gy, s); break;
+            case 2: launch_bulk<S, 64, 16>(q, grid, kc, gx, gy, s); break;
+            default: launch_bulk<S, 128, 8>(q, grid, kc, gx, gy, s); break;
+        }
         if (hipGetLastError() != hipSuccess) return -3;
     }
     if (mid && hipEventRecord(mid, s) != hipSuccess) return -3;

@@ -174,7 +174,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                                           int cj1, int slabs, int bx, int by, int kb, int ke,
                                           float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
     constexpr bool PML = MASK != 0;
-    static_assert(kBlk % TX == 0, "a tile row must lie inside one layout block");
+    // a wave (64 consecutive threads of a tile row) must lie inside one layout block: PML tiles
+    // divide a block; bulk tiles, which start on a block, may also span whole blocks
+    static_assert(kBlk % TX == 0 || (!PML && TX % kBlk == 0), "a wave must lie inside one layout block");
     using PR = Pair<S>;
     using P2 = typename PR::type;
     constexpr bool PREFETCH = sizeof(S) == 2;  // packed pairs leave room for the next plane
@@ -232,8 +234,10 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         q[m].template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)(k0 + FB3 * (m - 1)) * wpl), wrange), col, col1);
 
     // halo cells of this thread: cell c = tid + r*NT (r < HPT) of the list x-halo (4 columns x TY
-    // rows) then y-halo (4 rows x LX columns); a thread without a cell loads element 0 in vain
-    constexpr int NHX = 4 * TY, NHY = 4 * LX, NH = NHX + NHY;
+    // rows) then y-halo (4 rows x TX columns: the derivatives are one-dimensional, so the corner
+    // cells, which lie on the diagonal neighbours' lines, are never read); a thread without a
+    // cell loads element 0 in vain
+    constexpr int NHX = 4 * TY, NHY = 4 * TX, NH = NHX + NHY;
     constexpr int HPT = (NH + NT - 1) / NT;
     int hl[HPT];
     unsigned hcol[HPT], hcol1[HPT];
@@ -253,8 +257,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 hc = c4 < hx ? c4 : TX + c4;  // low-side columns [0,hx), high-side [hx+TX, LX)
             } else {
                 const int t = c - NHX;
-                const int r4 = t / LX;
-                hc = t % LX;
+                const int r4 = t / TX;
+                hc = hx + t % TX;
                 hr = r4 < hy ? r4 : TY + r4;
             }
             const int ig = i0 - hx + hc, jg = j0 - hy + hr;

@@ -69,8 +69,8 @@ __device__ __forceinline__ void fs_tile(const StageParams& p, int bx, int by, Fs
     const int kt = p.fk1 - 1;  // the surface plane
     const float rDH = p.rDH, DT = p.DT;
 
-    // halo cells of this thread (same enumeration as the bulk tile)
-    constexpr int NHX = 4 * TY, NHY = 4 * LX, NH = NHX + NHY;
+    // halo cells of this thread (same enumeration as the bulk tile: no corner cells)
+    constexpr int NHX = 4 * TY, NHY = 4 * TX, NH = NHX + NHY;
     constexpr int HPT = (NH + NT - 1) / NT;
     int hi_[HPT], hj_[HPT], hl[HPT];
 #pragma unroll
@@ -83,8 +83,8 @@ __device__ __forceinline__ void fs_tile(const StageParams& p, int bx, int by, Fs
             hc = c4 < hx ? c4 : TX + c4;
         } else if (c < NH) {
             const int tt = c - NHX;
-            const int r4 = tt / LX;
-            hc = tt % LX;
+            const int r4 = tt / TX;
+            hc = hx + tt % TX;
             hr = r4 < hy ? r4 : TY + r4;
         }
         hi_[r] = i0 - hx + hc;
