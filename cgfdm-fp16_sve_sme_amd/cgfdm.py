@@ -91,6 +91,7 @@ def lib():
         L.cgfdm_kernel_timing.argtypes = [vp, i, vp, P(ll)]
         L.cgfdm_enable_timing.argtypes = [vp, i]
         L.cgfdm_partition.argtypes = [vp, vp]
+        L.cgfdm_uniform_metric_pair.argtypes = [vp, P(i)]
         L.cgfdm_decompose.argtypes = [vp, vp, i]
         L.cgfdm_set_model_layered.argtypes = [vp, vp, i, vp, vp, vp, vp]
         L.cgfdm_finite_fault_map.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, ll,
@@ -233,6 +234,12 @@ class Domain:
         out = np.zeros(4, np.int64)
         _chk(lib().cgfdm_partition(self.h, _p(out)))
         return tuple(int(x) for x in out)
+
+    def uniform_metric_pair(self):
+        """True when the stage kernels take (xi_x, eta_y) from one stored copy."""
+        out = ctypes.c_int(0)
+        _chk(lib().cgfdm_uniform_metric_pair(self.h, ctypes.byref(out)))
+        return bool(out.value)
 
     def stage_timing(self, reset=False):
         avg = ctypes.c_double()

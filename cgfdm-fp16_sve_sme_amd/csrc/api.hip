@@ -170,6 +170,7 @@ long long estimate(const cgfdm_desc& d) {
     b += (long long)(d.NX + 6) * (d.NY + 6) * 18 * 4;
     b += 3LL * (d.NX + d.NY + d.NZ + 18) * 4;
     b += (long long)d.NX * d.NY * 2 * 4;
+    b += 256;  // uniform first metric pair
     return b;
 }
 
@@ -302,10 +303,12 @@ int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
     h->Crho = desc->Crho > 0 ? desc->Crho : (low ? 1.0e6 : 1.0);
     if (const char* e = std::getenv("CGFDM_BULK_CFG")) h->bulk_cfg = std::atoi(e);
     if (const char* e = std::getenv("CGFDM_BULK_KC")) h->bulk_kc = std::atoi(e);
+    if (const char* e = std::getenv("CGFDM_STREAM_PAIR0")) h->pair0Stream = std::atoi(e) != 0;
     int rc;
     for (int b = 0; b < 4; ++b)
         if ((rc = h->alloc(&h->W[b], (size_t)h->lay.size(9) * h->sb))) return rc;
     if ((rc = h->alloc(&h->cjm, (size_t)h->lay.size(13) * h->sb))) return rc;
+    if ((rc = h->alloc(&h->m0, 16))) return rc;
     // PML slabs: x1 y1 z1 x2 y2 z2
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
@@ -395,6 +398,7 @@ int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& ho
     SetupParams q = setup_params(h, dter);
     LCHK(launch_setup_medium(h->prec, q, dev, h->cjm, h->stream));
     LCHK(launch_setup_metric(h->prec, q, h->cjm, h->stream));
+    LCHK(launch_pair0_check(h->prec, q, h->cjm, h->m0, h->stream));
     if (h->d.free_surface) LCHK(launch_setup_rdz(q, dev, h->rdz, h->stream));
     if (h->d.check_cfl) {
         if (h->async) { cleanup(); set_err("the CFL check needs a synchronous domain"); return CGFDM_EINVAL; }
@@ -434,6 +438,7 @@ StageParams domain_stage_params(const cgfdm_domain* h) {
     p.A = h->W[0];
     p.T = h->W[3];
     p.cjm = h->cjm;
+    p.m0 = h->pair0Stream ? nullptr : h->m0;
     const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
     const int lens[3] = {h->nx_, h->ny_, h->nz_};
     for (int a = 0; a < 3; ++a) {
@@ -870,6 +875,17 @@ int cgfdm_decompose(const cgfdm_desc* desc, int* boxes, int max_boxes) {
     }
     for (int b = 0; b < q.nbox; ++b) put(2, -1, q.box[b]);
     return n;
+}
+
+int cgfdm_uniform_metric_pair(const cgfdm_domain* h, int* out) {
+    if (!h || !out) { set_err("null argument"); return CGFDM_EINVAL; }
+    if (!h->modelSet) { set_err("set_model must precede uniform_metric_pair"); return CGFDM_ESTATE; }
+    HIPCHK(hipSetDevice(h->dev));
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, h->m0, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *out = (flag != 0 && !h->pair0Stream) ? 1 : 0;
+    return CGFDM_OK;
 }
 
 int cgfdm_partition(const cgfdm_domain* h, long long* out) {

@@ -166,6 +166,7 @@ struct StageParams {
     const void* A;    // W at the start of the step (RK register m)
     void* T;          // RK accumulator t
     const void* cjm;  // 13 planes: xi_x..zt_z, J, mu, lambda, buoyancy
+    const void* m0;   // {int uniform, int, first metric pair}: the pair's value when uniform (or null)
     const float* pa[3];  // PML alpha per axis (padded length)
     const float* pb[3];  // 1/beta
     const float* pd[3];  // d/beta
@@ -224,6 +225,8 @@ enum Prec { P_F32 = 1, P_F16 = 2, P_BF16 = 3 };
 int launch_setup_medium(int prec, const SetupParams& q, const MediumSrc& ms, void* cjm, hipStream_t s);
 int launch_vpmax(const SetupParams& q, const MediumSrc& ms, unsigned* out, hipStream_t s);
 int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s);
+// Whether the first metric pair is the same at every interior point (m0: see setup.hip).
+int launch_pair0_check(int prec, const SetupParams& q, const void* cjm, void* m0, hipStream_t s);
 int launch_setup_rdz(const SetupParams& q, const MediumSrc& ms, float* rdz, hipStream_t s);
 int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 

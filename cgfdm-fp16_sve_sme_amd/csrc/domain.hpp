@@ -54,6 +54,8 @@ struct cgfdm_domain {
     double Cv = 1, Cs = 1, Crho = 1;
     void* W[4] = {nullptr, nullptr, nullptr, nullptr};  // A, B, C, T
     void* cjm = nullptr;
+    void* m0 = nullptr;     // 16 bytes: whether (xi_x, eta_y) is uniform, and its value
+    bool pair0Stream = false;  // read the first metric pair per point even when uniform
     float* prof = nullptr;  // [ax bx dx][ay by dy][az bz dz]
     float* rdz = nullptr;
     float* auxmem = nullptr;

@@ -281,13 +281,21 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     P2 M[4];
     [[maybe_unused]] W9<S> hvN[HPT];
     [[maybe_unused]] P2 MN[4];
+    // The first pair (xi_x, eta_y) comes from the copy in p.m0 while the set-up found it uniform
+    // over the grid: every lane and plane then loads the same cached line instead of streaming
+    // the pair. The choice is of a descriptor and an offset, so no load sits in a branch; the
+    // pair's load is cached (the copy is re-read by every wave).
+    const int uni = p.m0 ? __builtin_amdgcn_readfirstlane(*(const int*)p.m0) : 0;
+    const Rsrc rU = make_rsrc((const S*)(p.m0 ? (const char*)p.m0 + 8 : Cb), 2);
+    const unsigned colc0 = uni ? 0u : colc;
     auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[4]) {
         const Rsrc rW = make_rsrc((const S*)(Wb + (long long)kk * wpl), wrange);
 #pragma unroll
         for (int r = 0; r < HPT; ++r) h[r].template load<PB, CPH>(rW, hcol[r], hcol1[r]);
         const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kk * cpl), crange);
+        m[0] = PR::template ld<kCpDefault>(uni ? rU : rC, colc0, 0);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
+        for (int c = 1; c < 4; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
     };
     if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
     // PML slabs of the box, per normal axis a: the x/y profile values of the lane, and the

@@ -176,8 +176,17 @@ int cgfdm_enable_timing(cgfdm_domain* h, int on);
 
 /* How a stage is split: out[0] points of the bulk box, out[1] the other interior points (PML,
  * free-surface and general boxes),
- * out[2] bulk tile shape (0: 64x8, 1: 64x4, -1: not tuned yet), out[3] z-chunk (0: automatic). */
+ * out[2] bulk tile shape (0: 64x8, 1: 64x4, 2: 64x16, 3: 128x8, -1: not tuned yet), out[3] z-chunk
+ * (0: automatic). */
 int cgfdm_partition(const cgfdm_domain* h, long long* out);
+
+/* Whether the stage kernels take the first metric pair (xi_x, eta_y) from one stored copy:
+ * *out = 1 when set_model found the same stored pair at every interior point (the case on the
+ * terrain-following grids this library builds, where x depends on i only and y on j only), 0 when
+ * the pair is read per point. Results are the same either way. Environment CGFDM_STREAM_PAIR0=1
+ * (read at domain creation) forces the per-point reads. A diagnostic of this implementation, with
+ * no counterpart in the reference. Waits for the domain's stream. */
+int cgfdm_uniform_metric_pair(const cgfdm_domain* h, int* out);
 
 /* The decomposition of the interior a stage uses, from the descriptor alone (no device):
  * writes up to max_boxes records of 8 ints (i0, i1, j0, j1, k0, k1 in padded indices, kind,

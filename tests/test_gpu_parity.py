@@ -147,3 +147,30 @@ def test_low_precision_vs_fp64(gpu, oracle_lib, prec, tol):
     e = rel_l2(g.stations()[:, :, :3], o.stations()[:, :, :3])
     print(f"{prec} seismogram rel L2 vs FP64 oracle: {e:.3e}")
     assert e <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_uniform_metric_pair_matches_per_point_reads(gpu, monkeypatch, prec):
+    """On a terrain-following grid every interior point stores the same (xi_x, eta_y), and the
+    stage kernels then take that pair from one stored copy: the wavefield after some steps is
+    bit-for-bit the one of per-point reads (CGFDM_STREAM_PAIR0=1)."""
+    cg = gpu
+    NX, NY, NZ = 80, 44, 30
+    d = cg.make_desc(NX, NY, NZ, DH=100.0, DT=0.006, NT=10, nPML=6, precision=prec, use_terrain=1, depth=3000.0)
+    terrain = models.gauss_hill(NX, NY, 100.0, h0=400.0, width=1200.0)
+    # storage units: 16-bit domains store velocity x 5e3 and stress x 1e-6
+    scale = {"Cv": 5e3, "Cs": 1e-6} if prec == "fp16" else {}
+    w0 = models.smooth_state(NX, NY, NZ, seed=5, **scale).astype(np.float32)
+    states, flags = [], []
+    for stream in ("0", "1"):
+        monkeypatch.setenv("CGFDM_STREAM_PAIR0", stream)
+        g = cg.Domain(d)
+        g.set_model(terrain, *models.homogeneous(NX, NY, NZ))
+        g.set_state(w0)
+        g.step(4)
+        states.append(g.state())
+        flags.append(g.uniform_metric_pair())
+        g.close()
+    assert flags == [True, False]
+    assert np.isfinite(states[0]).all() and np.abs(states[0]).max() > 0
+    assert np.array_equal(states[0], states[1])
