@@ -1,76 +1,15 @@
 """Readers for the reference's per-rank output files and a restatement of its layered-medium
 rule, used to check the command-line driver's files.
 
-Partition: the first NX % PX ranks get one more point (init_grid.cpp:38-75,
-pyscripts/GRID.py:10-93). File layouts: station.cpp:382-405, PGV.cpp:103-111, data_io.cpp:323-431.
+The readers are the library's own (cgfdm_outputs: partition init_grid.cpp:38-75,
+pyscripts/GRID.py:10-93; layouts station.cpp:382-405, PGV.cpp:103-111, data_io.cpp:323-526), so
+the driver tests exercise them on the files the driver writes.
 """
-import os
 import struct
 
 import numpy as np
 
-
-def partition(N, P):
-    n = np.full(P, N // P)
-    res = N % P
-    n[:res] += 1
-    front = np.concatenate([[0], np.cumsum(n)[:-1]])
-    return n, front
-
-
-def read_stations(out_dir, params, NT):
-    """All station files -> (xyz[n,3], wave[n,NT,9]) in file order (rank order, then station order)."""
-    xyz, wave = [], []
-    for X in range(params["PX"]):
-        for Y in range(params["PY"]):
-            for Z in range(params["PZ"]):
-                fn = os.path.join(out_dir, f"station_mpi_{X}_{Y}_{Z}.bin")
-                if not os.path.exists(fn):
-                    continue
-                raw = open(fn, "rb").read()
-                n = len(raw) // (3 * 4 + NT * 9 * 4)
-                assert n * (12 + NT * 36) == len(raw), fn
-                xyz.append(np.frombuffer(raw[: 12 * n], np.int32).reshape(n, 3))
-                wave.append(np.frombuffer(raw[12 * n:], np.float32).reshape(n, NT, 9))
-    return np.concatenate(xyz), np.concatenate(wave)
-
-
-def read_pgv(out_dir, params):
-    NX, NY = params["NX"], params["NY"]
-    nx, fx = partition(NX, params["PX"])
-    ny, fy = partition(NY, params["PY"])
-    Z = params["PZ"] - 1
-    out = np.zeros((NY, NX, 2), np.float32)
-    for X in range(params["PX"]):
-        for Y in range(params["PY"]):
-            a = np.fromfile(os.path.join(out_dir, f"PGV_Z_mpi_{X}_{Y}_{Z}.bin"), np.float32)
-            out[fy[Y]:fy[Y] + ny[Y], fx[X]:fx[X] + nx[X]] = a.reshape(ny[Y], nx[X], 2)
-    return out
-
-
-def read_plane(out_dir, name, axis, params):
-    """Reassembles {name}_{X|Y|Z}_mpi_*.bin: axis 0 -> (NZ, NY), 1 -> (NZ, NX), 2 -> (NY, NX)."""
-    NX, NY, NZ = params["NX"], params["NY"], params["NZ"]
-    parts = [partition(NX, params["PX"]), partition(NY, params["PY"]), partition(NZ, params["PZ"])]
-    fast = 1 if axis == 0 else 0
-    slow = 1 if axis == 2 else 2
-    shape = {0: (NZ, NY), 1: (NZ, NX), 2: (NY, NX)}[axis]
-    out = np.full(shape, np.nan, np.float32)
-    tag = "XYZ"[axis]
-    found = 0
-    for X in range(params["PX"]):
-        for Y in range(params["PY"]):
-            for Z in range(params["PZ"]):
-                fn = os.path.join(out_dir, f"{name}_{tag}_mpi_{X}_{Y}_{Z}.bin")
-                if not os.path.exists(fn):
-                    continue
-                r = (X, Y, Z)
-                nf, ff = parts[fast][0][r[fast]], parts[fast][1][r[fast]]
-                ns, fs = parts[slow][0][r[slow]], parts[slow][1][r[slow]]
-                out[fs:fs + ns, ff:ff + nf] = np.fromfile(fn, np.float32).reshape(ns, nf)
-                found += 1
-    assert found > 0, name
-    return out
+from cgfdm_outputs import partition, read_pgv, read_plane, read_stations  # noqa: F401
 
 
 def write_layer_file(path, dep_m, vs, vp, rho):
