@@ -134,7 +134,7 @@ __global__ void k_source_multi(BatchDev b) {
 template <class S, int TX, int TY, int IRK>
 __global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b) {
     constexpr int irk = IRK;
-    __shared__ BulkLds<TX, TY> lds;
+    __shared__ BulkLds<S, TX, TY> lds;
     const int nEnt = b.tab->nBulk, nTiles = b.tab->nTiles;
     // the persistent grid is a multiple of the XCD count, so work index w keeps its hardware
     // block's XCD (w % kXcds) and xcd_order gives every XCD a contiguous run of tiles
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(TX* TY) k_bulk_multi(BatchDev b) {
 template <class S, int IRK, int MASK>
 __global__ void __launch_bounds__(pml_tx(MASK) * pml_ty(MASK)) k_pml_multi(BatchDev b) {
     constexpr int TX = pml_tx(MASK), TY = pml_ty(MASK), g = pml_group_of(MASK);
-    __shared__ BulkLds<TX, TY> lds;
+    __shared__ BulkLds<S, TX, TY> lds;
     const int nEnt = b.tab->nPml[g], nTiles = b.tab->nPmlTiles[g];
     for (int w = blockIdx.x; w < nTiles; w += gridDim.x) {
         const int t = (int)xcd_order(w, nTiles);

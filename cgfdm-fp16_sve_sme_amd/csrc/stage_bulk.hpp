@@ -39,9 +39,18 @@ __host__ __device__ __forceinline__ int pml_tile_x0(const Layout& L, int ci0, in
     return ci0 - ((ci0 + L.lead) & (TX - 1));
 }
 
-template <int TX, int TY> struct BulkLds {
-    static constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and var stride
-    float v[2][9 * VS];
+// LDS copy of a plane with its halo, double-buffered. FP32 storage: the 9 values as FP32, one
+// word per variable. 16-bit storage: the stored bits themselves, one 32-bit word per variable
+// pair (4 words) and the unpaired variable in the low half of a fifth word, so staging needs no
+// conversion and one LDS read brings both variables of a pair (the FMAs widen the halves; the
+// arithmetic is the same as on FP32 copies of these values).
+template <class S, int TX, int TY> struct BulkLds {
+    static constexpr int LX = TX + 4, LY = TY + 4, VS = LX * LY;  // LDS row pitch and word stride
+    static constexpr bool PACKED = sizeof(S) == 2;
+    static constexpr int NW = PACKED ? 5 : 9;  // words per cell
+    using word = std::conditional_t<PACKED, unsigned, float>;
+    using array = word[2][NW * VS];
+    array v;
 };
 
 // Cache policy of a buffer access: kCpNt marks a stream the kernel touches once (non-temporal:
@@ -144,6 +153,12 @@ template <class S> struct W9 {
     __device__ __forceinline__ float operator[](int v) const {
         return v == 8 ? Pair<S>::f1(s) : (v & 1) ? Pair<S>::hi(p[v >> 1]) : Pair<S>::lo(p[v >> 1]);
     }
+    // the stored bits as the packed LDS words of one cell (16-bit storage; word stride vs)
+    __device__ __forceinline__ void put(unsigned* L, int vs, int cell) const {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) L[c * vs + cell] = __builtin_bit_cast(unsigned, p[c]);
+        L[4 * vs + cell] = (unsigned)__builtin_bit_cast(unsigned short, s);
+    }
     // PB: bytes from one pair to the next; col/col1: byte offsets of pair 0 and of variable 8
     template <int PB, int CP = kCpDefault> __device__ __forceinline__ void load(Rsrc r, unsigned col, unsigned col1) {
 #pragma unroll
@@ -172,7 +187,7 @@ __device__ __forceinline__ void store9(Rsrc r, unsigned col, unsigned col1, cons
 template <class S, int TX, int TY, int IRK, int MASK = 0>
 __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2, int FB3, int ci0, int ci1, int cj0,
                                           int cj1, int slabs, int bx, int by, int kb, int ke,
-                                          float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+                                          typename BulkLds<S, TX, TY>::array& lds) {
     constexpr bool PML = MASK != 0;
     // a wave (64 consecutive threads of a tile row) must lie inside one layout block: PML tiles
     // divide a block; bulk tiles, which start on a block, may also span whole blocks
@@ -180,7 +195,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     using PR = Pair<S>;
     using P2 = typename PR::type;
     constexpr bool PREFETCH = sizeof(S) == 2;  // packed pairs leave room for the next plane
-    constexpr int LX = BulkLds<TX, TY>::LX, VS = BulkLds<TX, TY>::VS;
+    constexpr int LX = BulkLds<S, TX, TY>::LX, VS = BulkLds<S, TX, TY>::VS;
+    constexpr bool PACKED = BulkLds<S, TX, TY>::PACKED;
     constexpr int NT = TX * TY;
     constexpr int PB = 2 * kBlk * (int)sizeof(S);  // bytes from one variable pair to the next
     // cache policies (measured): the metric/medium values, the RK registers A and T and the
@@ -337,23 +353,30 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         const W9<S>& q3 = q[(3 + R) % 5];
         const W9<S>& q4 = q[(4 + R) % 5];
         const bool active = it < nk;  // iterations past the chunk store nothing
-        float* Lb = lds[it & 1];
+        auto* Lb = lds[it & 1];
         const int kc = min(max(k, 0), kmax);  // clamped plane (only the inactive iterations need it)
         const long long pw = (long long)kc * wpl;
         if constexpr (!PREFETCH) fetch_plane(kc, hv, M);
         // stage the centre plane and its halo in LDS
-        float cen[9];
+        [[maybe_unused]] float cen[9];
+        if constexpr (PACKED) {
+            q1.put(Lb, VS, lc);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
-            cen[v] = q1[v];
-            Lb[v * VS + lc] = cen[v];
-        }
+            for (int r = 0; r < HPT; ++r)
+                if (hown[r]) hv[r].put(Lb, VS, hl[r]);
+        } else {
 #pragma unroll
-        for (int r = 0; r < HPT; ++r)
-            if (hown[r]) {
-#pragma unroll
-                for (int v = 0; v < 9; ++v) Lb[v * VS + hl[r]] = hv[r][v];
+            for (int v = 0; v < 9; ++v) {
+                cen[v] = q1[v];
+                Lb[v * VS + lc] = cen[v];
             }
+#pragma unroll
+            for (int r = 0; r < HPT; ++r)
+                if (hown[r]) {
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) Lb[v * VS + hl[r]] = hv[r][v];
+                }
+        }
         __syncthreads();
         // requests for later iterations: the plane entering the queue and, with prefetch, the
         // next plane's halo and metric/medium pairs (clamped at the grid end; unused there)
@@ -412,10 +435,21 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             [[maybe_unused]] float draw[3][9];  // raw derivatives along the slab normals
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
-                const float* Lv = Lb + v * VS;
-                const float c = cen[v];
-                float d0 = cx0 * Lv[ox0] + cx1 * c + cx2 * Lv[ox2] + cx3 * Lv[ox3] + cx4 * Lv[ox4];
-                float d1 = cy0 * Lv[oy0] + cy1 * c + cy2 * Lv[oy2] + cy3 * Lv[oy3] + cy4 * Lv[oy4];
+                // value of variable v at LDS cell o
+                auto Lv = [&](int o) -> float {
+                    if constexpr (PACKED) {
+                        if (v == 8) return PR::f1(__builtin_bit_cast(typename PR::one, (unsigned short)Lb[4 * VS + o]));
+                        const P2 x = __builtin_bit_cast(P2, Lb[(v >> 1) * VS + o]);
+                        return (v & 1) ? PR::hi(x) : PR::lo(x);
+                    } else {
+                        return Lb[v * VS + o];
+                    }
+                };
+                float c;
+                if constexpr (PACKED) c = q1[v];
+                else c = cen[v];
+                float d0 = cx0 * Lv(ox0) + cx1 * c + cx2 * Lv(ox2) + cx3 * Lv(ox3) + cx4 * Lv(ox4);
+                float d1 = cy0 * Lv(oy0) + cy1 * c + cy2 * Lv(oy2) + cy3 * Lv(oy3) + cy4 * Lv(oy4);
                 float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
                 if constexpr ((MASK & 1) != 0) { draw[0][v] = d0; d0 *= pBeta[0]; }
                 if constexpr ((MASK & 2) != 0) { draw[1][v] = d1; d1 *= pBeta[1]; }
@@ -548,7 +582,7 @@ template <class S, int TX, int TY, int IRK>
 // left to itself the compiler gave one RK stage 129 VGPRs and occupancy 3 (6 % slower).
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_stage_bulk(StageParams p, int kc, int gx, int gy) {
-    __shared__ BulkLds<TX, TY> lds;
+    __shared__ BulkLds<S, TX, TY> lds;
     const long long t = xcd_order(blockIdx.x, (long long)gridDim.x);
     const int bx = (int)(t % gx), by = (int)((t / gx) % gy), bz = (int)(t / ((long long)gx * gy));
     const int kb = p.bk0 + bz * kc;
@@ -583,7 +617,7 @@ __device__ __forceinline__ int pml_entry(const PmlTiles& tl, int e0, int e1, int
 // Tile (bx, by, chunk) of entry e.
 template <class S, int TX, int TY, int IRK, int MASK>
 __device__ __forceinline__ void pml_tile(const StageParams& p, const PmlTiles& tl, int e, int r,
-                                         float (&lds)[2][9 * BulkLds<TX, TY>::VS]) {
+                                         typename BulkLds<S, TX, TY>::array& lds) {
     const int bx = r % tl.gx[e], by = (r / tl.gx[e]) % tl.gy[e], bz = r / (tl.gx[e] * tl.gy[e]);
     const int* B = p.pbox[tl.box[e]];
     const int kb = B[4] + bz * tl.kc[e];
@@ -593,7 +627,7 @@ __device__ __forceinline__ void pml_tile(const StageParams& p, const PmlTiles& t
 
 template <class S, int TX, int TY, int IRK, int MASK>
 __global__ void __launch_bounds__(TX* TY) k_stage_pml(StageParams p, PmlTiles tl) {
-    __shared__ BulkLds<TX, TY> lds;
+    __shared__ BulkLds<S, TX, TY> lds;
     constexpr int g = pml_group_of(MASK);
     const int e0 = tl.first[g], e1 = tl.first[g + 1];
     const int t = tl.start[e0] + (int)xcd_order(blockIdx.x, (long long)gridDim.x);
