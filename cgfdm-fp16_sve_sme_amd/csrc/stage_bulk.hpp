@@ -448,9 +448,27 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 float c;
                 if constexpr (PACKED) c = q1[v];
                 else c = cen[v];
-                float d0 = cx0 * Lv(ox0) + cx1 * c + cx2 * Lv(ox2) + cx3 * Lv(ox3) + cx4 * Lv(ox4);
-                float d1 = cy0 * Lv(oy0) + cy1 * c + cy2 * Lv(oy2) + cy3 * Lv(oy3) + cy4 * Lv(oy4);
-                float d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
+                float d0, d1, d2;
+                if constexpr (PACKED) {
+                    // FMA chains from the centre term (c converted once for the three axes), so
+                    // every other stored half enters a mixed-precision FMA without a conversion
+                    d0 = __builtin_fmaf(cx0, Lv(ox0), cx1 * c);
+                    d0 = __builtin_fmaf(cx2, Lv(ox2), d0);
+                    d0 = __builtin_fmaf(cx3, Lv(ox3), d0);
+                    d0 = __builtin_fmaf(cx4, Lv(ox4), d0);
+                    d1 = __builtin_fmaf(cy0, Lv(oy0), cy1 * c);
+                    d1 = __builtin_fmaf(cy2, Lv(oy2), d1);
+                    d1 = __builtin_fmaf(cy3, Lv(oy3), d1);
+                    d1 = __builtin_fmaf(cy4, Lv(oy4), d1);
+                    d2 = __builtin_fmaf(cz0, q0[v], cz1 * c);
+                    d2 = __builtin_fmaf(cz2, q2[v], d2);
+                    d2 = __builtin_fmaf(cz3, q3[v], d2);
+                    d2 = __builtin_fmaf(cz4, q4[v], d2);
+                } else {
+                    d0 = cx0 * Lv(ox0) + cx1 * c + cx2 * Lv(ox2) + cx3 * Lv(ox3) + cx4 * Lv(ox4);
+                    d1 = cy0 * Lv(oy0) + cy1 * c + cy2 * Lv(oy2) + cy3 * Lv(oy3) + cy4 * Lv(oy4);
+                    d2 = cz0 * q0[v] + cz1 * c + cz2 * q2[v] + cz3 * q3[v] + cz4 * q4[v];
+                }
                 if constexpr ((MASK & 1) != 0) { draw[0][v] = d0; d0 *= pBeta[0]; }
                 if constexpr ((MASK & 2) != 0) { draw[1][v] = d1; d1 *= pBeta[1]; }
                 if constexpr ((MASK & 4) != 0) { draw[2][v] = d2; d2 *= pBeta[2]; }
