@@ -65,6 +65,10 @@ def test_bad_descriptors_are_rejected(gpu):
     assert rc == -1 and "2^31" in msg
     h = ctypes.c_void_p()
     assert gpu.lib().cgfdm_domain_create(None, 0, ctypes.byref(h)) == -1
+    # queries on a null domain are argument errors, reported before any device call
+    flag = ctypes.c_int(7)
+    assert gpu.lib().cgfdm_uniform_metric_pair(None, ctypes.byref(flag)) == -1 and flag.value == 7
+    assert gpu.lib().cgfdm_partition(None, None) == -1
 
 
 def test_estimate_bytes_scales_with_precision(gpu):
