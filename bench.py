@@ -192,6 +192,8 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True)
             first = info["t_submit"] if first is None else min(first, info["t_submit"])
             last = max(last, info["t_done"])
             done += 1
+            if done % 32 == 0 and done < len(reqs):
+                log(f"  {done}/{len(reqs)} requests served, {time.perf_counter() - t0:.0f}s")
     th.join()
     stats = srv.stats()
     srv.close()
