@@ -184,7 +184,10 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
             kc = (nzb + nch - 1) / nch;
         }
         kc = std::max(1, std::min(kc, nzb));
+        // a requested length is the longest chunk: the planes are split evenly over as many
+        // chunks (a short last chunk would pay the queue's start-up planes for little work)
         const int nch = (nzb + kc - 1) / kc;
+        kc = (nzb + nch - 1) / nch;
         const unsigned grid = (unsigned)((long long)gx * gy * nch);
         switch (cfg) {
             case 0: launch_bulk<S, 64, 8>(q, grid, kc, gx, gy, s); break;
