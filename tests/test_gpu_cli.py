@@ -126,3 +126,33 @@ def test_cli_finite_fault_matches_oracle(tmp_path, gpu, oracle_lib):
     ref = o.stations()[order]
     assert np.abs(ref).max() > 0
     assert rel_l2(wave, ref) <= 1e-5
+
+
+def test_cli_output_compare_fp16_vs_fp32(tmp_path, gpu):
+    """The output tooling on the driver's own files: the same Gauss-hill run with FP32 and FP16
+    storage, compared as the reference's comparison scripts do (cgfdm_outputs.compare): zero
+    against itself, and FP16 within a few percent of FP32 pointwise (of the largest value)."""
+    import cgfdm_outputs as co
+    NX, NY, NZ, DH, DT = 40, 36, 30, 100.0, 0.008
+    nsteps = 60
+    params = {"TMAX": nsteps * DT + 1e-4, "DT": DT, "DH": DH, "NX": NX, "NY": NY, "NZ": NZ,
+              "PX": 2, "PY": 1, "PZ": 1, "centerX": NX // 2, "centerY": NY // 2, "IT_SKIP": 50,
+              "sliceX": 10, "sliceY": 12, "sliceZ": 20, "sliceFreeSurf": 1, "nPML": 8,
+              "gauss_hill": 1, "useTerrain": 0, "useMedium": 0, "useMultiSource": 0,
+              "useSingleSource(ricker)": 1, "rickerfc": 2.0, "sourceX": 20, "sourceY": 18,
+              "sourceZ": 24, "Depth(km)": 9.9}
+    st = [[5, 5, NZ - 1], [20, 18, NZ - 1], [33, 30, NZ - 1], [12, 25, NZ - 1]]
+    (tmp_path / "station.json").write_text(json.dumps({"station(point)": {f"S{q}": s for q, s in enumerate(st)}}))
+    for prec in ("fp32", "fp16"):
+        (tmp_path / "params.json").write_text(json.dumps(dict(params, out=f"out_{prec}")))
+        r = subprocess.run([CLI, "--precision", prec], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+    a, b = str(tmp_path / "out_fp32"), str(tmp_path / "out_fp16")
+    same = co.compare(a, a, params, var="Vz", it=50)
+    assert same["plane"]["max_relative_error_percent"] == 0.0
+    assert same["stations"]["max_abs_difference"] == 0.0 and same["pgv"]["intensity_cells_differing"] == 0
+    res = co.compare(a, b, params, var="Vz", it=50)
+    assert res["plane"]["max_abs_standard"] > 0 and res["stations"]["n"] == len(st)
+    assert 0 < res["plane"]["max_relative_error_percent"] < 2.0
+    assert 0 < res["stations"]["max_relative_error_percent"] < 2.0
+    assert res["pgv"]["max_relative_error_percent"] < 2.0
