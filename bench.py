@@ -142,7 +142,7 @@ def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=5):
                       f"{steps} timed steps, OpenMP over k on {threads} threads"}
 
 
-def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True):
+def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True, least_work_first=False):
     """Serves `reqs` [(arrival_s, request)] with one cgfdm.Server; requests are submitted at
     their arrival times by a second thread (paced) or all at once. Returns latencies (s, from
     submission to results on the host), the makespan and the server counters."""
@@ -153,7 +153,7 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True)
         descs.append(cgfdm.make_desc(NX, NY, NZ, DH=r["DH"], DT=r["DT"], NT=r["NT"], nPML=r["nPML"],
                                      precision=r["precision"]))
     srv = cgfdm.Server(precision=reqs[0][1]["precision"], max_domains=max_domains,
-                       steps_per_launch=steps_per_launch, use_graph=use_graph)
+                       steps_per_launch=steps_per_launch, use_graph=use_graph, least_work_first=least_work_first)
     ids = {}
     err = []
 
@@ -201,14 +201,16 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True)
     return np.array(lat), last - first, wall, stats
 
 
-def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batched", "one_at_a_time")):
+def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batched", "one_at_a_time"),
+            least_work_first=False):
     reqs = wl.serving_stream(n, seed)
     lups = sum(r["grid"][0] * r["grid"][1] * r["grid"][2] * r["NT"] for _, r in reqs)
     out = {}
     for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
         if name not in modes:
             continue
-        lat, span, wall, st = serving_run(reqs, md, steps_per_launch, use_graph=use_graph)
+        lat, span, wall, st = serving_run(reqs, md, steps_per_launch, use_graph=use_graph,
+                                          least_work_first=least_work_first)
         out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
                      "glups": round(lups / span / 1e9, 3), "peak_resident": st["peak_resident"],
@@ -217,6 +219,7 @@ def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batc
             f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s, {lups / span / 1e9:.2f} GLUPS, "
             f"peak resident {st['peak_resident']}")
     out["requests"] = n
+    out["admit_order"] = "least work first" if least_work_first else "arrival"
     out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at 20/s, "
                      f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
                      f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface "
@@ -263,6 +266,8 @@ def main():
     ap.add_argument("--serve-steps-per-launch", type=int, default=8)
     ap.add_argument("--serve-graph", type=int, default=1, help="1: replay captured HIP graphs, 0: launch directly")
     ap.add_argument("--serve-modes", default="batched,one_at_a_time", help="comma list of batched, one_at_a_time")
+    ap.add_argument("--serve-order", default="arrival", choices=("arrival", "least-work"),
+                    help="which waiting request a free slot takes")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -286,7 +291,7 @@ def main():
             res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
         del model
     serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph),
-                    tuple(args.serve_modes.split(","))) \
+                    tuple(args.serve_modes.split(",")), args.serve_order == "least-work") \
         if args.serve_requests > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

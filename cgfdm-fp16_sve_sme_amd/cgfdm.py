@@ -292,7 +292,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
                 ("hbm_budget", ctypes.c_longlong), ("host_budget", ctypes.c_longlong),
                 ("max_domains", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("admit_order", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -333,11 +333,11 @@ class Server:
     table=(xyz, rate[nt][npts][6], dt); stations (n x 3)."""
 
     def __init__(self, precision=FP16, device=0, hbm_budget=0, host_budget=0, max_domains=16,
-                 steps_per_launch=8, use_graph=True):
+                 steps_per_launch=8, use_graph=True, least_work_first=False):
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         c = ServerConfig(device, precision, hbm_budget, host_budget, max_domains, steps_per_launch,
-                         1 if use_graph else 0)
+                         1 if use_graph else 0, 1 if least_work_first else 0)
         h = ctypes.c_void_p()
         _chk(lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)))
         self.h = h

@@ -4,8 +4,9 @@
 // budget), one pinned host arena and a batch of up to kMaxSlots resident domains. A worker
 // thread runs the loop
 //
-//   admit:   take submitted requests in arrival order while a slot is free and the arena has a
-//            block for the request's whole device footprint; set the domain up in that block
+//   admit:   take submitted requests (in arrival order, or the least work first) while a slot is
+//            free and the arena has a block for the request's whole device footprint; set the
+//            domain up in that block
 //            (metric, medium, sources, stations: all enqueued, no host wait) and write its slot;
 //   advance: replay a HIP graph of `steps_per_launch` whole time steps over every resident
 //            domain (batch.hip), or launch the same kernels directly;
@@ -318,15 +319,23 @@ void cgfdm_server::loop() {
     long long n = 0;
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        // admit in arrival order
+        // admit in arrival order, or the least work first
         while (!pending.empty() && !freeSlots.empty()) {
-            Req& head = *pending.front();
+            size_t pick = 0;
+            if (cfg.admit_order == 1) {
+                auto work = [](const Req& r) {
+                    return (long long)r.desc.NX * r.desc.NY * r.desc.NZ * (long long)r.desc.NT;
+                };
+                for (size_t q = 1; q < pending.size(); ++q)
+                    if (work(*pending[q]) < work(*pending[pick])) pick = q;
+            }
+            Req& head = *pending[pick];
             size_t doff, hoff;
             if (!devArena.alloc(head.devBytes, doff)) break;
             if (!hostArena.alloc(head.hostBytes, hoff)) { devArena.release(doff, head.devBytes); break; }
             Resident R;
-            R.req = std::move(pending.front());
-            pending.pop_front();
+            R.req = std::move(pending[pick]);
+            pending.erase(pending.begin() + (long)pick);
             R.devOff = doff;
             R.hostOff = hoff;
             R.slot = freeSlots.back();
@@ -432,6 +441,7 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         return CGFDM_EINVAL;
     }
     if (c->steps_per_launch < 1 || c->steps_per_launch > 64) { set_err("steps_per_launch must be in [1, 64]"); return CGFDM_EINVAL; }
+    if (c->admit_order != 0 && c->admit_order != 1) { set_err("admit_order must be 0 or 1"); return CGFDM_EINVAL; }
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail("hipSetDevice", e);
     auto S = std::make_unique<cgfdm_server>();

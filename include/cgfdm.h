@@ -228,6 +228,9 @@ typedef struct cgfdm_server_config {
     int max_domains;        /* resident simulations, 1..64                                */
     int steps_per_launch;   /* whole time steps per batch launch, 1..64                   */
     int use_graph;          /* 1: replay a captured HIP graph; 0: launch directly         */
+    int admit_order;        /* which waiting request a free slot takes: 0 the earliest
+                               submitted, 1 the least work (NX*NY*NZ*NT) — a lower median
+                               latency when more requests wait than slots                 */
 } cgfdm_server_config;
 
 typedef struct cgfdm_request {

@@ -69,6 +69,10 @@ def test_bad_descriptors_are_rejected(gpu):
     flag = ctypes.c_int(7)
     assert gpu.lib().cgfdm_uniform_metric_pair(None, ctypes.byref(flag)) == -1 and flag.value == 7
     assert gpu.lib().cgfdm_partition(None, None) == -1
+    # a server configuration with an unknown admission order is refused before any device call
+    c = gpu.ServerConfig(0, gpu.FP16, 0, 0, 4, 8, 1, 2)
+    assert gpu.lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert "admit_order" in gpu.lib().cgfdm_last_error().decode()
 
 
 def test_estimate_bytes_scales_with_precision(gpu):

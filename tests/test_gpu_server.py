@@ -194,3 +194,41 @@ def test_layered_model_on_device_equals_host(gpu):
         assert np.array_equal(a.cjm(), b.cjm()), use_terrain
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("least_work_first", [False, True])
+def test_admission_order(gpu, least_work_first):
+    """One slot, held by a long request while two more wait: the free slot takes the earlier
+    submitted one in arrival order, the one with less work (NX*NY*NZ*NT) when asked."""
+    cg = gpu
+    first, a, b = _requests(cg, cg.FP16, seed=5, n=3)  # point, table, point sources
+    first["desc"].NT = 120
+    a["desc"].NT = 16  # within its table's length
+    b["desc"].NT = 64
+
+    def work(r):
+        d = r["desc"]
+        return d.NX * d.NY * d.NZ * d.NT
+
+    big, small = (a, b) if work(a) > work(b) else (b, a)
+    infos = {}
+    with cg.Server(precision=cg.FP16, max_domains=1, steps_per_launch=4, least_work_first=least_work_first) as s:
+        ids = {s.submit(first): "first"}
+        t0 = time.time()
+        while s.stats()["resident"] < 1:
+            assert time.time() - t0 < 30, "the first request never became resident"
+            time.sleep(0.001)
+        ids[s.submit(big)] = "big"
+        ids[s.submit(small)] = "small"
+        while len(infos) < 3:
+            got = s.wait(timeout_ms=60000)
+            assert got, f"no request completed within 60 s ({s.stats()})"
+            for rid in got:
+                rc, st, pgv, info = s.result(rid)
+                assert rc == 0, info["error"]
+                infos[ids[rid]] = info
+    assert infos["first"]["t_done"] <= min(infos["big"]["t_admit"], infos["small"]["t_admit"])
+    if least_work_first:
+        assert infos["small"]["t_admit"] < infos["big"]["t_admit"]
+    else:
+        assert infos["big"]["t_admit"] < infos["small"]["t_admit"]
