@@ -32,12 +32,13 @@ class Desc(ctypes.Structure):
         ("centerX", ctypes.c_int), ("centerY", ctypes.c_int),
         ("Cv", ctypes.c_double), ("Cs", ctypes.c_double), ("Crho", ctypes.c_double),
         ("check_cfl", ctypes.c_int),
+        ("chi_mode", ctypes.c_int),
     ]
 
 
 def make_desc(NX, NY, NZ, DH=100.0, DT=0.008, NT=100, nPML=12, precision=FP32,
               free_surface=1, pml=1, use_terrain=0, depth=None, centerX=None, centerY=None,
-              Cv=0.0, Cs=0.0, Crho=0.0, check_cfl=0):
+              Cv=0.0, Cs=0.0, Crho=0.0, check_cfl=0, chi_mode=0):
     if isinstance(precision, str):
         precision = PRECISIONS[precision]
     d = Desc()
@@ -47,7 +48,7 @@ def make_desc(NX, NY, NZ, DH=100.0, DT=0.008, NT=100, nPML=12, precision=FP32,
     d.depth = DH * NZ if depth is None else depth
     d.centerX = NX // 2 if centerX is None else centerX
     d.centerY = NY // 2 if centerY is None else centerY
-    d.Cv, d.Cs, d.Crho, d.check_cfl = Cv, Cs, Crho, check_cfl
+    d.Cv, d.Cs, d.Crho, d.check_cfl, d.chi_mode = Cv, Cs, Crho, check_cfl, chi_mode
     return d
 
 

@@ -146,6 +146,7 @@ int validate(const cgfdm_desc* d) {
         }
     }
     if (d->use_terrain && !(d->depth != 0.0)) { set_err("terrain stretch needs a nonzero depth"); return CGFDM_EINVAL; }
+    if (d->chi_mode != 0 && d->chi_mode != 1) { set_err("chi_mode must be 0 or 1"); return CGFDM_EINVAL; }
     return 0;
 }
 
@@ -220,6 +221,7 @@ SetupParams setup_params(const cgfdm_domain* h, const double* dterrain) {
     q.terrain = dterrain;
     q.fs = h->d.free_surface;
     q.Cv = h->Cv; q.Cs = h->Cs; q.Crho = h->Crho;
+    q.chi_mode = h->d.chi_mode;
     return q;
 }
 

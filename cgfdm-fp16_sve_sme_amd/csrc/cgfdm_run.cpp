@@ -14,8 +14,11 @@
 //   {OUT}/coord?_*, Vs_*, Vp_*, rho_* model planes (data_io.cpp:497-526)
 //
 // Usage: cgfdm_run [--params params.json] [--stations station.json] [--precision fp32|fp16|bf16]
-//                  [--device N] [--model-only]
+//                  [--device N] [--model-only] [--chi 2lambda+mu|lambda+2mu]
 // --model-only writes the model planes and stops before any device call.
+// --chi selects the coefficient of the free-surface matrices: the reference's 2*lambda + mu
+// (contravariant.cpp:266, the default, for parity) or lambda + 2*mu, the traction-free condition;
+// the run prints which one it uses.
 // Exit status follows the reference's abort codes where a status can carry them: 100 no
 // params.json, 101 unparseable params.json (reference 1001), 110 CFL violation, 180 no source.
 #include <cctype>
@@ -291,6 +294,7 @@ int main(int argc, char** argv) {
     std::string paramsPath = "params.json", stationPath = "station.json", precision = "fp32";
     int device = 0;
     bool modelOnly = false;
+    int chiMode = 0;
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
         auto next = [&]() -> std::string { return a + 1 < argc ? argv[++a] : ""; };
@@ -299,6 +303,12 @@ int main(int argc, char** argv) {
         else if (s == "--precision") precision = next();
         else if (s == "--device") device = std::atoi(next().c_str());
         else if (s == "--model-only") modelOnly = true;
+        else if (s == "--chi") {
+            const std::string v = next();
+            if (v == "2lambda+mu") chiMode = 0;
+            else if (v == "lambda+2mu") chiMode = 1;
+            else { std::fprintf(stderr, "--chi takes 2lambda+mu or lambda+2mu\n"); return 2; }
+        }
         else { std::fprintf(stderr, "unknown argument %s\n", s.c_str()); return 2; }
     }
     Params P;
@@ -322,6 +332,7 @@ int main(int argc, char** argv) {
     d.depth = P.Depth * 1000.0;
     d.centerX = P.centerX; d.centerY = P.centerY;
     d.check_cfl = 1;
+    d.chi_mode = chiMode;
 
     // coordinates and terrain (coord.cpp:106-307)
     std::vector<double> terrain((size_t)nx_ * ny_, 0.0);
@@ -434,6 +445,8 @@ int main(int argc, char** argv) {
         std::printf("model written to %s (%d stations)\n", P.OUT.c_str(), nst);
         return 0;
     }
+    std::printf("free surface: chi = %s\n", chiMode ? "lambda + 2 mu (traction-free, --chi lambda+2mu)"
+                                                   : "2 lambda + mu (the reference's, contravariant.cpp:266)");
     cgfdm_domain* h = nullptr;
     int rc = cgfdm_domain_create(&d, device, &h);
     if (rc) { std::fprintf(stderr, "create: %s\n", cgfdm_last_error()); return 1; }

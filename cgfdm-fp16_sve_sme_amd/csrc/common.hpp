@@ -207,6 +207,7 @@ struct SetupParams {
     const double* terrain;  // padded (ny_, nx_)
     int fs;
     double Cv, Cs, Crho;
+    int chi_mode;  // 0: chi = 2*lambda + mu (reference), 1: lambda + 2*mu
 };
 
 // Source of the medium for the setup kernels: 3-D arrays on the interior grid (vs != null), or a

@@ -162,12 +162,15 @@ extern "C" int cgfdm_finite_fault_map(const char* path, const cgfdm_desc* desc, 
         y[i] = (float)(s * std::cos(az));
     }
 
-    // srcCoord2Index: depths relative to the shallowest point; node column by rounding toward
-    // zero of x/DH + 0.5 from the projection centre (centerX, centerY); node layer = the cell
-    // [z_k - dz1, z_k + dz2) (half distances to the neighbouring layers) holding the point, the
-    // last matching layer if rounding lets two match
+    // srcCoord2Index, the branch the reference compiles (NO_SOURCE_SMOOTH is defined in
+    // macro.h:76; multiSource.cpp:243-264): depths relative to the shallowest point; node column by
+    // rounding toward zero of x/DH + 0.5 from the projection centre (centerX, centerY), kept only
+    // inside the interior columns [HALO, nx+HALO) x [HALO, ny+HALO); node layer = the interior
+    // layer k in [HALO, nz+HALO) whose cell [z_k - dz1, z_k + dz2) (half distances to the
+    // neighbouring layers) holds the point, the last matching layer if rounding lets two match.
+    // Points outside these ranges are dropped without an error, as the reference drops them.
     const int H = 3;  // halo width of the padded grid (reference macro.h HALO)
-    const int nx_ = desc->NX + 2 * H, ny_ = desc->NY + 2 * H, nz_ = desc->NZ + 2 * H;
+    const int nx_ = desc->NX + 2 * H, ny_ = desc->NY + 2 * H;
     const float DH = (float)desc->DH;
     const float Depth = (float)desc->depth;
     float maxZ = -Depth;
@@ -182,30 +185,15 @@ extern "C" int cgfdm_finite_fault_map(const char* path, const cgfdm_desc* desc, 
         return (float)(k - H + 1) * DH - (float)desc->NZ * DH;
     };
     std::map<long long, long long> point_index;  // point -> node (padded linear index)
-    std::vector<int> px(npts, -1), py(npts, -1), pz(npts, -1);
     for (int i = 0; i < npts; ++i) {
         const int sx = int(x[i] / DH + 0.5) + desc->centerX + H;
         const int sy = int(y[i] / DH + 0.5) + desc->centerY + H;
-        if (sx < 0 || sx >= nx_ || sy < 0 || sy >= ny_) continue;
-        for (int k = 0; k < nz_; ++k) {
-            double dz1, dz2;
+        if (sx < H || sx >= desc->NX + H || sy < H || sy >= desc->NY + H) continue;
+        for (int k = H; k < desc->NZ + H; ++k) {
             const float zk = coord_z(sx, sy, k);
-            if (k == 0) {
-                dz2 = (coord_z(sx, sy, 1) - zk) * 0.5;
-                dz1 = dz2;
-            } else if (k + 1 == nz_) {
-                dz1 = (zk - coord_z(sx, sy, nz_ - 2)) * 0.5;
-                dz2 = dz1;
-            } else {
-                dz1 = (zk - coord_z(sx, sy, k - 1)) * 0.5;
-                dz2 = (coord_z(sx, sy, k + 1) - zk) * 0.5;
-            }
-            if (zk - dz1 <= zf[i] && zf[i] < dz2 + zk) {
-                point_index[i] = ((long long)k * ny_ + sy) * nx_ + sx;
-                px[i] = sx;
-                py[i] = sy;
-                pz[i] = k;
-            }
+            const double dz1 = (zk - coord_z(sx, sy, k - 1)) * 0.5;
+            const double dz2 = (coord_z(sx, sy, k + 1) - zk) * 0.5;
+            if (zk - dz1 <= zf[i] && zf[i] < dz2 + zk) point_index[i] = ((long long)k * ny_ + sy) * nx_ + sx;
         }
     }
 

@@ -202,7 +202,8 @@ __global__ void k_pair0_check(SetupParams q, const S* __restrict__ cjm, const vo
 }
 
 // Free-surface matrices -rDZ^-1 DX and -rDZ^-1 DY at the surface layer
-// (contravariant.cpp:257-351, including the reference's chi = 2*lambda + mu).
+// (contravariant.cpp:257-351). chi is the reference's 2*lambda + mu (contravariant.cpp:266) unless
+// the descriptor asks for lambda + 2*mu, the coefficient of the traction-free condition.
 __global__ void k_rdz(SetupParams q, MediumSrc ms, float* __restrict__ rdz) {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)q.nx * q.ny) return;
@@ -215,7 +216,7 @@ __global__ void k_rdz(SetupParams q, MediumSrc ms, float* __restrict__ rdz) {
     medium_at(q, vs, vp, rho, mu, lambda, buoy);
     const double xi_x = m[0], xi_y = m[1], xi_z = m[2], et_x = m[3], et_y = m[4], et_z = m[5];
     const double zt_x = m[6], zt_y = m[7], zt_z = m[8];
-    const double chi = 2.0 * lambda + mu, lam_mu = lambda + mu;
+    const double chi = q.chi_mode ? lambda + 2.0 * mu : 2.0 * lambda + mu, lam_mu = lambda + mu;
     double DZ[9], DX[9], DY[9], rD[9];
     DZ[0] = chi * zt_x * zt_x + mu * (zt_y * zt_y + zt_z * zt_z);
     DZ[1] = lam_mu * zt_x * zt_y;

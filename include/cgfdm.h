@@ -60,6 +60,9 @@ typedef struct cgfdm_desc {
     int centerX, centerY;  /* coordinate origin in grid indices (params centerX/Y)   */
     double Cv, Cs, Crho;   /* scale factors; 0 selects the defaults of the precision */
     int check_cfl;         /* 1: reject DT > 1.34*Hmin/Vpmax (calc_CFL.cpp:331-352)  */
+    int chi_mode;          /* free-surface matrices: 0 the reference's chi = 2*lambda + mu
+                              (contravariant.cpp:266), 1 chi = lambda + 2*mu, the traction-free
+                              condition; the two agree where lambda = mu (Vp/Vs = sqrt 3)      */
 } cgfdm_desc;
 
 typedef struct cgfdm_domain cgfdm_domain;

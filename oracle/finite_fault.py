@@ -121,7 +121,7 @@ def finite_fault_map(path, desc, terrain, lon0, lat0):
     f32 = np.float32
     rec, nt, dt = read_fault(path)
     npts = rec.shape[0]
-    nx_, ny_, nz_ = desc.NX + 2 * HALO, desc.NY + 2 * HALO, desc.NZ + 2 * HALO
+    nx_, ny_ = desc.NX + 2 * HALO, desc.NY + 2 * HALO
     DH = f32(desc.DH)
     # projTrans (multiSource.cpp:109-173)
     x = np.zeros(npts, np.float32)
@@ -134,22 +134,17 @@ def finite_fault_map(path, desc, terrain, lon0, lat0):
     zmax = max(f32(-f32(desc.depth)), z.max()) if npts else f32(0)
     z = (z - zmax).astype(np.float32)
     point_node = {}
+    # the NO_SOURCE_SMOOTH branch (macro.h:76 defines it; multiSource.cpp:243-264): interior
+    # columns [HALO, nx+HALO) and interior layers [HALO, nz+HALO) only, other points dropped
     for p in range(npts):
         sx = int(float(f32(x[p] / DH)) + 0.5) + desc.centerX + HALO
         sy = int(float(f32(y[p] / DH)) + 0.5) + desc.centerY + HALO
-        if not (0 <= sx < nx_ and 0 <= sy < ny_):
+        if not (HALO <= sx < desc.NX + HALO and HALO <= sy < desc.NY + HALO):
             continue
-        for k in range(nz_):
+        for k in range(HALO, desc.NZ + HALO):
             zk = coord_z(desc, terrain, sx, sy, k)
-            if k == 0:
-                dz2 = float(f32(coord_z(desc, terrain, sx, sy, 1) - zk)) * 0.5
-                dz1 = dz2
-            elif k + 1 == nz_:
-                dz1 = float(f32(zk - coord_z(desc, terrain, sx, sy, nz_ - 2))) * 0.5
-                dz2 = dz1
-            else:
-                dz1 = float(f32(zk - coord_z(desc, terrain, sx, sy, k - 1))) * 0.5
-                dz2 = float(f32(coord_z(desc, terrain, sx, sy, k + 1) - zk)) * 0.5
+            dz1 = float(f32(zk - coord_z(desc, terrain, sx, sy, k - 1))) * 0.5
+            dz2 = float(f32(coord_z(desc, terrain, sx, sy, k + 1) - zk)) * 0.5
             if float(zk) - dz1 <= float(z[p]) < dz2 + float(zk):
                 point_node[p] = (k * ny_ + sy) * nx_ + sx
     # readSourceParams + solveMomentRate (:365-545), points in increasing order

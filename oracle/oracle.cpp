@@ -215,7 +215,8 @@ struct Setup {
                     double et_x = cjm[3 * num + idx], et_y = cjm[4 * num + idx], et_z = cjm[5 * num + idx];
                     double zt_x = cjm[6 * num + idx], zt_y = cjm[7 * num + idx], zt_z = cjm[8 * num + idx];
                     double mu = cjm[10 * num + idx], lambda = cjm[11 * num + idx];
-                    double chi = 2.0 * lambda + mu; /* contravariant.cpp:266 */
+                    /* contravariant.cpp:266; chi_mode 1 is the opt-in traction-free lambda + 2 mu */
+                    double chi = d.chi_mode ? lambda + 2.0 * mu : 2.0 * lambda + mu;
                     double lam_mu = lambda + mu;
                     double DZ[9], DX[9], DY[9], rDZ[9];
                     DZ[0] = chi * zt_x * zt_x + mu * (zt_y * zt_y + zt_z * zt_z);
@@ -1018,6 +1019,26 @@ int orc_debug_rhs(void* hp, int FB1, int FB2, int FB3, double* out) {
                             ST<S>::get(s.hW[(size_t)v * s.num + s.INDEX(i + HALO, j + HALO, k + HALO)]);
         return 0;
     });
+}
+
+/* The auxiliary increments h_Aux of PML slab s (0..5: x1 y1 z1 x2 y2 z2, struct.h:565-574) as the
+ * last orc_debug_rhs left them (pml_deriv.cpp:176-207 plus pml_freeSurface.cpp:81-116): 9 values
+ * per slab point, [pos][9] with pos = i0 + j0*n0 + k0*n0*n1 (pml_deriv.cpp:164, 313, 464).
+ * Returns the number of slab points (0 if the slab is absent) or a negative status if cap is
+ * smaller. */
+long long orc_debug_aux_rhs(void* hp, int s, double* out, long long cap) {
+    Handle* h = static_cast<Handle*>(hp);
+    long long res = 0;
+    int rc = dispatch(h, [&](auto& sim) {
+        if (s < 0 || s > 5) { g_err = "slab index out of range"; return (int)CGFDM_EINVAL; }
+        const auto& X = sim.aux[s];
+        if (!X.on) { res = 0; return 0; }
+        if (cap < X.n * 9) { g_err = "output too small"; return (int)CGFDM_EINVAL; }
+        for (long long q = 0; q < X.n * 9; ++q) out[q] = (double)X.h[q];
+        res = X.n;
+        return 0;
+    });
+    return rc ? rc : res;
 }
 
 /* Applies only this step's source injection into W (the first action of propagate.cpp:145-150)

@@ -48,6 +48,8 @@ def lib():
         L.orc_current_step.argtypes = [vp]
         L.orc_debug_rhs.argtypes = [vp, i, i, i, vp]
         L.orc_debug_source.argtypes = [vp]
+        L.orc_debug_aux_rhs.restype = ll
+        L.orc_debug_aux_rhs.argtypes = [vp, i, vp, ll]
         L.orc_last_error.restype = ctypes.c_char_p
         L.orc_apply_L_line.argtypes = [vp, i, i, d, vp]
         L.orc_wave_rk.argtypes = [i, vp, vp, vp, vp, ll]
@@ -162,6 +164,21 @@ class Oracle:
         out = np.zeros((9, d.NZ, d.NY, d.NX), np.float64)
         self._chk(lib().orc_debug_rhs(self.h, FB1, FB2, FB3, _p(out)))
         return out
+
+    def aux_rhs(self, slab):
+        """Auxiliary increments h_Aux = DT*RHS of PML slab `slab` (0..5: x1 y1 z1 x2 y2 z2) as the
+        last rhs() call left them, shaped (n2, n1, n0, 9) in slab-local (k0, j0, i0)."""
+        d = self.desc
+        n = d.nPML
+        dims = [(d.NZ, d.NY, n), (d.NZ, n, d.NX), (n, d.NY, d.NX)][slab % 3]
+        cap = 9 * dims[0] * dims[1] * dims[2]
+        out = np.zeros(cap, np.float64)
+        r = lib().orc_debug_aux_rhs(self.h, slab, _p(out), cap)
+        if r < 0:
+            raise RuntimeError(f"oracle error {r}: {lib().orc_last_error().decode()}")
+        if r == 0:
+            return None
+        return out.reshape(dims + (9,))
 
     def inject_source(self):
         """Applies only this step's source injection into W."""

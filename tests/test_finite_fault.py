@@ -185,3 +185,56 @@ def test_library_aeqd_matches_oracle(cg):
 def test_missing_file_is_an_error(cg, tmp_path):
     with pytest.raises(cg.CgfdmError):
         cg.finite_fault_map(tmp_path / "none.bin", _desc(cg), None, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("use_terrain", [0, 1])
+def test_points_outside_the_interior_are_dropped(tmp_path, cg, use_terrain):
+    """The reference compiles srcCoord2Index with NO_SOURCE_SMOOTH (macro.h:76,
+    multiSource.cpp:243-264): only interior columns [HALO, N+HALO) and interior layers are kept.
+    Points in the halo ring of columns, below the bottom interior layer or above the top one are
+    dropped, not mapped onto halo nodes (which cgfdm_set_source_table would reject)."""
+    d = _desc(cg, NX=20, NY=18, NZ=16, use_terrain=use_terrain)
+    DH = d.DH
+    # terrain: a basin 250 m deep at the centre column, so the shallowest point (depth 0 by
+    # definition) lies above the surface there
+    terrain = models.gauss_hill(d.NX, d.NY, DH, h0=-250.0, width=300.0) if use_terrain else None
+    lon0, lat0 = 0.0, 0.0
+    R = 6378137.0
+
+    def lon_of(X):  # column X (interior index) on the equator
+        return np.degrees((X - d.centerX) * DH / R)
+
+    zc = -400.0  # the inside points, relative depth 0 once the shallowest point is subtracted
+    # int(x/DH + 0.5) rounds toward zero, so west of the centre a point at column X lands on
+    # column X+1: -2 and -4 land on the halo columns -1 and -3
+    cols = [d.centerX, -2, -4, d.NX, d.NX + 2, d.centerX, d.centerX + 2]
+    depth = [zc, zc - 100, zc - 100, zc - 100, zc - 100,
+             zc - (d.NZ + 1) * DH,   # one layer below the bottom interior layer
+             zc - (d.NZ + 2) * DH]   # two layers below
+    if use_terrain:
+        cols.append(d.centerX + 1)
+        depth.append(zc - 50.0)
+    cols.append(d.centerX)
+    depth.append(zc - 600.0)  # inside on either grid
+    n = len(cols)
+    lon = np.array([lon_of(X) for X in cols])
+    lat = np.zeros(n)
+    nt = 2
+    path = tmp_path / "edge.bin"
+    ff.write_fault(path, lon, lat, np.array(depth), np.full(n, 1e4), np.zeros(n), np.full(n, 90.0),
+                   np.zeros((n, nt)), np.ones((n, nt)), 0.1)
+    xo, ro, _ = ff.finite_fault_map(path, d, terrain, lon0, lat0)
+    xg, rg, _ = cg.finite_fault_map(path, d, terrain, lon0, lat0)
+    np.testing.assert_array_equal(xg, xo)
+    np.testing.assert_allclose(rg, ro, rtol=2e-6)
+    lim = np.array([d.NX, d.NY, d.NZ])
+    assert ((xo >= 0) & (xo < lim)).all(), xo
+    got = [tuple(int(c) for c in v) for v in xo]
+    if use_terrain:
+        # the basin: the shallowest point and its neighbour 50 m down sit above the surface there
+        # (surface at about -250 m, layers 90 m apart) and are dropped; the point 600 m down
+        # lands 350 m below the surface, four layers under the top
+        assert got == [(d.centerX, d.centerY, d.NZ - 5)]
+    else:
+        # the top layer of the centre column and the point six layers below it
+        assert got == [(d.centerX, d.centerY, d.NZ - 7), (d.centerX, d.centerY, d.NZ - 1)]
