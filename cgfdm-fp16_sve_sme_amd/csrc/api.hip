@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "domain.hpp"
+#include "stage_geom.hpp"
 
 using namespace cg;
 
@@ -117,17 +118,6 @@ int cgfdm_domain::settle() {
 
 namespace {
 
-long long slab_points(const cgfdm_desc& d, int s) {
-    if (!d.pml) return 0;
-    if (s == 5 && d.free_surface) return 0;
-    const long long n = d.nPML;
-    switch (s % 3) {
-        case 0: return n * d.NY * d.NZ;
-        case 1: return n * d.NX * d.NZ;
-        default: return n * d.NX * d.NY;
-    }
-}
-
 int validate(const cgfdm_desc* d) {
     if (!d) { set_err("null descriptor"); return CGFDM_EINVAL; }
     if (d->NX < 1 || d->NY < 1 || d->NZ < 4) { set_err("grid too small (NX,NY >= 1, NZ >= 4)"); return CGFDM_EINVAL; }
@@ -148,19 +138,6 @@ int validate(const cgfdm_desc* d) {
     if (d->use_terrain && !(d->depth != 0.0)) { set_err("terrain stretch needs a nonzero depth"); return CGFDM_EINVAL; }
     if (d->chi_mode != 0 && d->chi_mode != 1) { set_err("chi_mode must be 0 or 1"); return CGFDM_EINVAL; }
     return 0;
-}
-
-// Internal layout (Layout in common.hpp): rows padded to whole 64-column blocks and shifted so
-// that the first column of the bulk box (3 + nPML) starts a block.
-Layout make_layout(const cgfdm_desc& d) {
-    Layout L{};
-    L.nx_ = d.NX + 6; L.ny_ = d.NY + 6; L.nz_ = d.NZ + 6;
-    const int bi0 = HALO + (d.pml ? d.nPML : 0);
-    L.lead = (kBlk - bi0 % kBlk) % kBlk;
-    L.px = (L.lead + L.nx_ + kBlk - 1) / kBlk * kBlk;
-    L.nb = L.px / kBlk;
-    L.plane = (long long)L.px * L.ny_;
-    return L;
 }
 
 long long estimate(const cgfdm_desc& d) {
@@ -225,8 +202,6 @@ SetupParams setup_params(const cgfdm_domain* h, const double* dterrain) {
     return q;
 }
 
-void* stage_in(cgfdm_domain* h, int irk) { return h->W[irk == 0 ? 0 : (irk == 2 ? 2 : 1)]; }
-void* stage_out(cgfdm_domain* h, int irk) { return h->W[irk == 0 ? 1 : (irk == 1 ? 2 : (irk == 2 ? 1 : 0))]; }
 
 // ------------------------------------------------------------------ bulk launch-shape tuner
 // Candidate shapes of the bulk kernel: tile (64x8 or 64x4 columns) and z-chunk length (0 picks
@@ -315,23 +290,7 @@ int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
     if (auxTotal > 0 && (rc = h->alloc((void**)&h->auxmem, (size_t)auxTotal * 9 * 3 * sizeof(float)))) return rc;
-    long long off = 0;
-    for (int s = 0; s < 6; ++s) {
-        Slab& X = h->slab[s];
-        X.n = slab_points(h->d, s);
-        X.on = X.n > 0;
-        X.axis = s % 3;
-        const int nAxis = X.axis == 0 ? h->nx : (X.axis == 1 ? h->ny : h->nz);
-        X.lo = s < 3 ? HALO : HALO + nAxis - h->d.nPML;
-        X.n0 = X.axis == 0 ? h->d.nPML : h->nx;
-        X.n1 = X.axis == 1 ? h->d.nPML : h->ny;
-        if (X.on) {
-            X.a = h->auxmem + off;
-            X.m = X.a + 9 * X.n;
-            X.t = X.m + 9 * X.n;
-            off += 27 * X.n;
-        }
-    }
+    assign_slabs(h);
     if ((rc = h->alloc((void**)&h->prof, sizeof(float) * 3 * (h->nx_ + h->ny_ + h->nz_)))) return rc;
     if ((rc = h->alloc((void**)&h->rdz, sizeof(float) * 18 * (size_t)h->nx_ * h->ny_))) return rc;
     if ((rc = h->alloc((void**)&h->pgv, sizeof(float) * 2 * (size_t)h->nx * h->ny))) return rc;
@@ -435,34 +394,6 @@ int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& ho
     return CGFDM_OK;
 }
 
-StageParams domain_stage_params(const cgfdm_domain* h) {
-    StageParams p{};
-    p.A = h->W[0];
-    p.T = h->W[3];
-    p.cjm = h->cjm;
-    p.m0 = h->pair0Stream ? nullptr : h->m0;
-    const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
-    const int lens[3] = {h->nx_, h->ny_, h->nz_};
-    for (int a = 0; a < 3; ++a) {
-        p.pa[a] = h->prof + offs[a];
-        p.pb[a] = h->prof + offs[a] + lens[a];
-        p.pd[a] = h->prof + offs[a] + 2 * lens[a];
-    }
-    p.rdz = h->rdz;
-    for (int s = 0; s < 6; ++s) p.slab[s] = h->slab[s];
-    p.lay = h->lay;
-    p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
-    p.nx_ = h->nx_; p.ny_ = h->ny_;
-    p.nPML = h->d.nPML;
-    p.fs = h->d.free_surface;
-    p.pml = h->d.pml;
-    p.bulk_cfg = h->bulk_cfg < 0 ? 1 : h->bulk_cfg;
-    p.bulk_kc = h->bulk_kc;
-    p.rDH = (float)(1.0 / h->d.DH);
-    p.DT = (float)h->d.DT;
-    p.Crho = (float)h->Crho;
-    return p;
-}
 
 }  // namespace cg
 

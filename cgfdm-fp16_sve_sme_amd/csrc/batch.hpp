@@ -3,6 +3,7 @@
 #include "common.hpp"
 #include "stage_bulk.hpp"
 #include "stage_fs.hpp"
+#include "stage_geom.hpp"
 
 namespace cg {
 
@@ -68,8 +69,6 @@ struct BatchDev {
 
 struct BatchGrid { int bulkBlocks, genBlocks, pmlBlocks[kNumPmlMasks], fsBlocks; };
 
-void stage_boxes(StageParams& q, int use_fast);
-PmlTiles pml_tiles(const StageParams& q, int want);
 int batch_write_slot(const BatchDev& b, int s, const SlotDev& sd, const StageParams& base, hipStream_t st);
 int batch_clear_slot(const BatchDev& b, int s, hipStream_t st);
 int batch_step(int prec, int cfg, const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk);

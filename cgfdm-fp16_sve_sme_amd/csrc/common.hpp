@@ -241,7 +241,6 @@ int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 struct StageFork { hipStream_t side; hipEvent_t fork, join; };
 int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid,
                  const StageFork* fk = nullptr);
-void stage_boxes(StageParams& q, int use_fast);
 
 // Point lists (sources, stations) carry padded (i, j, k) triples.
 int launch_point_source(int prec, void* A, void* B, void* C, const Layout& L, int SX, int SY, int SZ,

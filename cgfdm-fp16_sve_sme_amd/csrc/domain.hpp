@@ -14,6 +14,7 @@
 
 #include "../../include/cgfdm.h"
 #include "common.hpp"
+#include "stage_geom.hpp"
 
 namespace cg {
 
@@ -58,7 +59,7 @@ struct cgfdm_domain {
     bool pair0Stream = false;  // read the first metric pair per point even when uniform
     float* prof = nullptr;  // [ax bx dx][ay by dy][az bz dz]
     float* rdz = nullptr;
-    float* auxmem = nullptr;
+    float* auxmem = nullptr;  // PML auxiliaries of every slab (FP32)
     cg::Slab slab[6]{};
     float* pgv = nullptr;
     // stations (padded i, j, k triples)
@@ -103,7 +104,65 @@ namespace cg {
 int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device);
 int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& host);
 int domain_bulk_tiling(const cgfdm_domain* h, int cfg, int& gx, int& gy, int& nch, int& kc);
-StageParams domain_stage_params(const cgfdm_domain* h);
+
+// Slab geometry and auxiliary registers of the six PML slabs (x1 y1 z1 x2 y2 z2), carved in that
+// order from h->auxmem (three FP32 registers of 9 variables each). h->d, h->nx..nz and h->auxmem
+// must be set; the slab sizes are slab_points.
+inline void assign_slabs(cgfdm_domain* h) {
+    long long off = 0;
+    for (int s = 0; s < 6; ++s) {
+        Slab& X = h->slab[s];
+        X.n = slab_points(h->d, s);
+        X.on = X.n > 0;
+        X.axis = s % 3;
+        const int nAxis = X.axis == 0 ? h->nx : (X.axis == 1 ? h->ny : h->nz);
+        X.lo = s < 3 ? HALO : HALO + nAxis - h->d.nPML;
+        X.n0 = X.axis == 0 ? h->d.nPML : h->nx;
+        X.n1 = X.axis == 1 ? h->d.nPML : h->ny;
+        if (X.on) {
+            X.a = h->auxmem + off;
+            X.m = X.a + 9 * X.n;
+            X.t = X.m + 9 * X.n;
+            off += 27 * X.n;
+        }
+    }
+}
+
+// Wavefield registers of RK stage irk: input and output (A -> B -> C -> B -> A).
+inline void* stage_in(const cgfdm_domain* h, int irk) { return h->W[irk == 0 ? 0 : (irk == 2 ? 2 : 1)]; }
+inline void* stage_out(const cgfdm_domain* h, int irk) {
+    return h->W[irk == 0 ? 1 : (irk == 1 ? 2 : (irk == 2 ? 1 : 0))];
+}
+
+// Parameters of a stage of domain h (stage registers, FB signs and irk are set per stage).
+inline StageParams domain_stage_params(const cgfdm_domain* h) {
+    StageParams p{};
+    p.A = h->W[0];
+    p.T = h->W[3];
+    p.cjm = h->cjm;
+    p.m0 = h->pair0Stream ? nullptr : h->m0;
+    const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
+    const int lens[3] = {h->nx_, h->ny_, h->nz_};
+    for (int a = 0; a < 3; ++a) {
+        p.pa[a] = h->prof + offs[a];
+        p.pb[a] = h->prof + offs[a] + lens[a];
+        p.pd[a] = h->prof + offs[a] + 2 * lens[a];
+    }
+    p.rdz = h->rdz;
+    for (int s = 0; s < 6; ++s) p.slab[s] = h->slab[s];
+    p.lay = h->lay;
+    p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
+    p.nx_ = h->nx_; p.ny_ = h->ny_;
+    p.nPML = h->d.nPML;
+    p.fs = h->d.free_surface;
+    p.pml = h->d.pml;
+    p.bulk_cfg = h->bulk_cfg < 0 ? 1 : h->bulk_cfg;
+    p.bulk_kc = h->bulk_kc;
+    p.rDH = (float)(1.0 / h->d.DH);
+    p.DT = (float)h->d.DT;
+    p.Crho = (float)h->Crho;
+    return p;
+}
 long long domain_estimate(const cgfdm_desc& d);
 int validate_desc(const cgfdm_desc* d);
 

@@ -596,9 +596,10 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 // z chunks of kc planes), dealt to the XCDs in contiguous runs (xcd_order). One instantiation per
 // RK stage.
 template <class S, int TX, int TY, int IRK>
-// Occupancy 4 waves per SIMD (at most 128 VGPRs): two 64x8 workgroups per CU, as the LDS allows;
+// Occupancy at least 4 waves per SIMD (at most 128 VGPRs; the launch bound's second argument is
+// the minimum number of waves per execution unit): two 64x8 workgroups per CU, as the LDS allows;
 // left to itself the compiler gave one RK stage 129 VGPRs and occupancy 3 (6 % slower).
-__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(TX* TY, 4)
 k_stage_bulk(StageParams p, int kc, int gx, int gy) {
     __shared__ BulkLds<S, TX, TY> lds;
     const long long t = xcd_order(blockIdx.x, (long long)gridDim.x);

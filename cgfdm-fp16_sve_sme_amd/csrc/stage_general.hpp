@@ -286,4 +286,20 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
     rk_update<S>(p, i, j, k, h);
 }
 
+// General kernel over a list of boxes (flattened with prefix sums).
+template <class S>
+__global__ void __launch_bounds__(256) k_stage_general(StageParams p) {
+    const long long t = xcd_order(blockIdx.x, (long long)gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= p.boxStart[p.nbox]) return;
+    int b = 0;
+    while (b + 1 < p.nbox && t >= p.boxStart[b + 1]) ++b;
+    const long long r = t - p.boxStart[b];
+    const int* B = p.box[b];
+    const int nxb = B[1] - B[0], nyb = B[3] - B[2];
+    const int i = B[0] + (int)(r % nxb);
+    const int j = B[2] + (int)((r / nxb) % nyb);
+    const int k = B[4] + (int)(r / ((long long)nxb * nyb));
+    stage_point_general<S>(p, i, j, k);
+}
+
 }  // namespace cg
