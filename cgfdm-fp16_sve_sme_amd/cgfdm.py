@@ -88,6 +88,7 @@ def lib():
                   "cgfdm_get_cjm"):
             getattr(L, n).argtypes = [vp, vp]
         L.cgfdm_get_slice.argtypes = [vp, i, i, i, vp]
+        L.cgfdm_get_slice_noalloc.argtypes = [vp, i, i, i, vp]
         L.cgfdm_stage_timing.argtypes = [vp, i, P(ctypes.c_double), P(ll)]
         L.cgfdm_kernel_timing.argtypes = [vp, i, vp, P(ll)]
         L.cgfdm_enable_timing.argtypes = [vp, i]
@@ -198,6 +199,14 @@ class Domain:
         shape = {0: (d.NZ, d.NY), 1: (d.NZ, d.NX), 2: (d.NY, d.NX)}[axis]
         out = np.zeros(shape, np.float32)
         _chk(lib().cgfdm_get_slice(self.h, axis, index, var, _p(out)))
+        return out
+
+    def slice_noalloc(self, axis, index, var):
+        """As slice(), staged in the library's fixed device plane (cgfdm_get_slice_noalloc)."""
+        d = self.desc
+        shape = {0: (d.NZ, d.NY), 1: (d.NZ, d.NX), 2: (d.NY, d.NX)}[axis]
+        out = np.zeros(shape, np.float32)
+        _chk(lib().cgfdm_get_slice_noalloc(self.h, axis, index, var, _p(out)))
         return out
 
     def state(self):

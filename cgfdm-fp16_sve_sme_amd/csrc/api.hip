@@ -62,6 +62,31 @@ namespace {
 const int FB_TABLE[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
                             {-1, 1, -1},  {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};  // propagate.cpp:107-117
 
+// Staging plane of cgfdm_get_slice_noalloc: one file-scope device array per GPU, 16 MiB (any
+// plane of at most 4 Mi points, e.g. 2048 x 2048), held by every GPU the library runs on until the
+// process ends. In host code `scratch` names a host-side stand-in, not a device address: its
+// device address is looked up once per device with hipGetSymbolAddress (slice_scratch) and only
+// that address is passed to kernels and copies.
+constexpr long long kSliceScratchFloats = 4LL << 20;
+__device__ __align__(256) float scratch[kSliceScratchFloats];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;               // guards the table and serialises the calls that use it
+float* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device
+
+// Device address of `scratch` on the current device `dev` (looked up on the first call there).
+// Called with g_scratch_mu held.
+hipError_t slice_scratch(int dev, float** out) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[dev]) {
+        float* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ domain memory
@@ -690,6 +715,28 @@ int cgfdm_get_slice(cgfdm_domain* h, int axis, int index, int var, float* out) {
     HIPCHK(hipMemcpyAsync(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->release(tmp);
+    return CGFDM_OK;
+}
+
+int cgfdm_get_slice_noalloc(cgfdm_domain* h, int axis, int index, int var, float* out) {
+    if (!h || !out || axis < 0 || axis > 2 || var < 0 || var > 8) { set_err("bad slice request"); return CGFDM_EINVAL; }
+    const int lim = axis == 0 ? h->nx : (axis == 1 ? h->ny : h->nz);
+    if (index < 0 || index >= lim) { set_err("slice index %d outside [0,%d)", index, lim); return CGFDM_EINVAL; }
+    const long long n = axis == 0 ? (long long)h->ny * h->nz : (axis == 1 ? (long long)h->nx * h->nz : (long long)h->nx * h->ny);
+    if (n > kSliceScratchFloats) {
+        set_err("slice of %lld points exceeds the %lld-point staging plane of cgfdm_get_slice_noalloc", n,
+                kSliceScratchFloats);
+        return CGFDM_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    HIPCHK(hipSetDevice(h->dev));
+    float* tmp = nullptr;
+    HIPCHK(slice_scratch(h->dev, &tmp));
+    const float c = (float)(1.0 / (var < 3 ? h->Cv : h->Cs));
+    // the kernel writes all n values, so nothing left by an earlier call is read
+    LCHK(launch_slice(h->prec, h->W[0], h->lay, h->nx, h->ny, h->nz, axis, index, var, c, tmp, h->stream));
+    HIPCHK(hipMemcpyAsync(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return CGFDM_OK;
 }
 

@@ -157,6 +157,15 @@ int cgfdm_get_pgv(cgfdm_domain* h, float* out);
  *  axis 2: Z = index, out NY x NX. */
 int cgfdm_get_slice(cgfdm_domain* h, int axis, int index, int var, float* out);
 
+/* cgfdm_get_slice without any device allocation: the plane is staged in a fixed 16 MiB device
+ * array of the library (one per GPU, kept until the process ends) instead of a buffer allocated
+ * per call, so callers that cannot allocate (tight snapshot loops, code under HIP graph capture,
+ * harnesses that forbid hipMalloc) can take snapshots. Planes above 4 Mi points (4194304) are
+ * rejected with CGFDM_EINVAL. Calls in one process are serialised on a lock (the staging array is
+ * shared by every domain on a GPU and holds the last call's plane); the call waits for the
+ * domain's stream. No counterpart in the reference. Same output as cgfdm_get_slice. */
+int cgfdm_get_slice_noalloc(cgfdm_domain* h, int axis, int index, int var, float* out);
+
 /* Raw state access for tests and restarts: the 9 wavefield variables on the interior
  * grid (9 x NZ x NY x NX float, x fastest) in storage units (scaled by Cv/Cs). */
 int cgfdm_get_state(cgfdm_domain* h, float* out);
