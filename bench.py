@@ -110,7 +110,7 @@ def src_k(NZ, DH):
     return NZ - 1 - int(round(5000.0 / DH))
 
 
-def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=5):
+def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=10):
     """The FP32 CPU restatement of the reference scalar path (oracle), OpenMP over k-slabs on
     the host cores, timed on a z-slab sample of the same workload recipe."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -137,9 +137,12 @@ def cpu_baseline(DH, DT, sample=(1001, 1001, 41), warmup=2, steps=5):
     log(f"[cpu] {NX}x{NY}x{NZ} x {steps} steps on {threads} threads: {wall:.2f}s, {glups:.4f} GLUPS")
     o.close()
     return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "per_core_mlups": round(glups * 1e3 / threads, 4),
+            "reference_per_core_mlups": 0.72,
             "sample": f"{NX}x{NY}x{NZ} z-slab of the same recipe (terrain, layered+5% medium, "
                       f"double-couple table source, stations, PGV), FP32, {warmup} warm-up + "
-                      f"{steps} timed steps, OpenMP over k on {threads} threads"}
+                      f"{steps} timed steps, OpenMP over k on {threads} threads; the reference's own "
+                      f"FP32-CGFDM rate is 0.72 MLUPS per ARM core (64 ranks, 601x601x151 Tibet run)"}
 
 
 def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True, least_work_first=False):
@@ -174,7 +177,7 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True,
     th = threading.Thread(target=feeder)
     t0 = time.perf_counter()
     th.start()
-    lat, first, last = [], None, 0.0
+    lat, queue, service, first, last = [], [], [], None, 0.0
     done = 0
     while done < len(reqs):
         if err:
@@ -189,6 +192,8 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True,
             if not np.isfinite(st).all():
                 raise RuntimeError(f"request {rid}: non-finite seismograms")
             lat.append(info["t_done"] - info["t_submit"])
+            queue.append(info["t_admit"] - info["t_submit"])
+            service.append(info["t_done"] - info["t_admit"])
             first = info["t_submit"] if first is None else min(first, info["t_submit"])
             last = max(last, info["t_done"])
             done += 1
@@ -198,7 +203,7 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True,
     stats = srv.stats()
     srv.close()
     wall = time.perf_counter() - t0
-    return np.array(lat), last - first, wall, stats
+    return np.array(lat), last - first, wall, stats, np.array(queue), np.array(service)
 
 
 def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batched", "one_at_a_time"),
@@ -209,23 +214,79 @@ def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batc
     for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
         if name not in modes:
             continue
-        lat, span, wall, st = serving_run(reqs, md, steps_per_launch, use_graph=use_graph,
-                                          least_work_first=least_work_first)
+        lat, span, wall, st, qt, sv = serving_run(reqs, md, steps_per_launch, use_graph=use_graph,
+                                                  least_work_first=least_work_first)
         out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
                      "glups": round(lups / span / 1e9, 3), "peak_resident": st["peak_resident"],
-                     "launches": st["launches"]}
+                     "launches": st["launches"],
+                     "queue_p50_s": round(float(np.percentile(qt, 50)), 3),
+                     "queue_p99_s": round(float(np.percentile(qt, 99)), 3),
+                     "service_p50_s": round(float(np.percentile(sv, 50)), 3),
+                     "service_p99_s": round(float(np.percentile(sv, 99)), 3)}
         log(f"[serving {name}] {n} requests in {span:.1f}s: {n / span:.3f} solves/s, p50 "
-            f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s, {lups / span / 1e9:.2f} GLUPS, "
+            f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s (queued p50 {np.percentile(qt, 50):.2f}s, "
+            f"service p50 {np.percentile(sv, 50):.2f}s p99 {np.percentile(sv, 99):.2f}s), {lups / span / 1e9:.2f} GLUPS, "
             f"peak resident {st['peak_resident']}")
     out["requests"] = n
     out["admit_order"] = "least work first" if least_work_first else "arrival"
     out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at 20/s, "
                      f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
-                     f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16, 5x5 surface "
+                     f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16 with scales chosen per request from the source, 5x5 surface "
                      f"stations")
     if "batched" in out and "one_at_a_time" in out:
         out["speedup"] = round(out["batched"]["solves_per_s"] / out["one_at_a_time"]["solves_per_s"], 3)
+    return out
+
+
+def finite_fault_run(steps, prec="fp16", DH=100.0):
+    """The kinematic finite-fault workload (workloads.finite_fault_workload: 1201 x 1201 x 401,
+    80 x 20 subfaults written as a reference-layout source file and mapped by
+    cgfdm_finite_fault_map, fractal terrain, perturbed layered medium, 400 stations, PGV), FP16
+    storage with scales chosen from the source; `steps` steps (4000 = the whole 32 s), progress
+    every 500. Returns GLUPS, ms per step and the peak ground velocity."""
+    import tempfile
+    t0 = time.time()
+    w = wl.finite_fault_workload(DH=DH)
+    NX, NY, NZ = w["grid"]
+    log(f"[finite fault] model {NX}x{NY}x{NZ} built in {time.time() - t0:.1f}s")
+    d32 = cgfdm.make_desc(NX, NY, NZ, DH=DH, DT=w["DT"], NT=steps, nPML=w["nPML"], use_terrain=1, depth=w["depth"])
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "fault.bin")
+        wl.write_rupture(path, w["rupture"])
+        xyz, table, tdt = cgfdm.finite_fault_map(path, d32, w["terrain"], w["lon0"], w["lat0"])
+    vs, vp, rho = w["medium"]
+    sc = {}
+    if prec != "fp32":
+        Cv, Cs = cgfdm.source_scales(DH, xyz, table, tdt, float((rho.astype(np.float64) * vs * vs).max()))
+        sc = {"Cv": Cv, "Cs": Cs, "Crho": 1e6}
+    d = cgfdm.make_desc(NX, NY, NZ, DH=DH, DT=w["DT"], NT=steps, nPML=w["nPML"], precision=prec, use_terrain=1,
+                        depth=w["depth"], check_cfl=1, **sc)
+    dom = cgfdm.Domain(d, device=0)
+    dom.set_model(w["terrain"], vs, vp, rho)
+    dom.set_source_table(xyz, table, tdt)
+    dom.set_stations(w["stations"])
+    dom.sync()
+    log(f"[finite fault] {len(xyz)} source nodes, scales {sc}, device bytes {dom.bytes / 2**30:.1f} GiB")
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(500, steps - done)
+        dom.step(n)
+        dom.sync()
+        done += n
+        log(f"[finite fault] {done}/{steps} steps, {time.perf_counter() - t0:.1f}s")
+    wall = time.perf_counter() - t0
+    pgv = dom.pgv()
+    st = dom.stations()
+    dom.close()
+    glups = NX * NY * NZ * steps / wall / 1e9
+    out = {"glups": round(glups, 4), "ms_per_step": round(wall / steps * 1e3, 3), "steps": steps,
+           "grid": [NX, NY, NZ], "storage": prec, "source_nodes": int(len(xyz)),
+           "pgv_max_m_s": round(float(pgv[:, :, 1].max()), 4), "finite": bool(np.isfinite(st).all() and
+                                                                               np.isfinite(pgv).all())}
+    log(f"[finite fault] {steps} steps in {wall:.1f}s: {glups:.3f} GLUPS, PGV max {out['pgv_max_m_s']} m/s, "
+        f"finite={out['finite']}")
     return out
 
 
@@ -259,7 +320,7 @@ def main():
     ap.add_argument("--grid", default="1001,1001,401")
     ap.add_argument("--precisions", default="fp16,fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving stream to run (0: skip; 512: the whole stream)")
     ap.add_argument("--serve-slots", type=int, default=32)
@@ -268,6 +329,9 @@ def main():
     ap.add_argument("--serve-modes", default="batched,one_at_a_time", help="comma list of batched, one_at_a_time")
     ap.add_argument("--serve-order", default="arrival", choices=("arrival", "least-work"),
                     help="which waiting request a free slot takes")
+    ap.add_argument("--finite-fault-steps", type=int, default=0,
+                    help="also run the kinematic finite-fault workload (1201x1201x401, FP16) for this many "
+                         "steps (4000: the whole rupture) and report it under 'finite_fault'")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -290,6 +354,7 @@ def main():
         for prec in precs:
             res[prec] = run_gpu(prec, grid, DH, DT, args.steps, args.warmup, model)
         del model
+    ffault = finite_fault_run(args.finite_fault_steps) if args.finite_fault_steps > 0 else None
     serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph),
                     tuple(args.serve_modes.split(",")), args.serve_order == "least-work") \
         if args.serve_requests > 0 else None
@@ -336,6 +401,8 @@ def main():
         out[f"{prec}_store"] = {k: (round(x, 4) if isinstance(x, float) else x) for k, x in v.items()}
     if "fp16" in res and "fp32" in res:
         out["fp16_over_fp32"] = round(res["fp16"]["glups"] / res["fp32"]["glups"], 3)
+    if ffault:
+        out["finite_fault"] = ffault
     if serve and "batched" in serve:
         out["latency_p50_ms"] = round(serve["batched"]["p50_s"] * 1e3, 1)
         out["latency_p99_ms"] = round(serve["batched"]["p99_s"] * 1e3, 1)

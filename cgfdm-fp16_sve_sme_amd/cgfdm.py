@@ -99,6 +99,8 @@ def lib():
         L.cgfdm_finite_fault_map.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, ll,
                                              P(ll), P(i), P(f)]
         L.cgfdm_aeqd_forward.argtypes = [ctypes.c_double, ctypes.c_double, ll, vp, vp, vp, vp]
+        L.cgfdm_source_scales.argtypes = [ctypes.c_double, vp, vp, ll, i, f, ctypes.c_double, P(ctypes.c_double),
+                                          P(ctypes.c_double)]
         L.cgfdm_server_create.argtypes = [vp, P(vp)]
         L.cgfdm_server_submit.argtypes = [vp, vp, P(ll)]
         L.cgfdm_server_wait.argtypes = [vp, vp, i, i]
@@ -296,7 +298,23 @@ def estimate_bytes(desc):
     return lib().cgfdm_estimate_bytes(ctypes.byref(desc))
 
 
-# ------------------------------------------------------------------ serving
+def source_scales(DH, xyz=None, table=None, dt=None, mu_max=None):
+    """(Cv, Cs) of 16-bit storage chosen from the source (cgfdm_source_scales): the point source
+    without a table, else the moment-rate table (nt x npts x 6) at the global interior indices
+    xyz (npts x 3), sampled every dt s, in a medium whose largest shear modulus is mu_max."""
+    cv, cs = ctypes.c_double(), ctypes.c_double()
+    if table is None:
+        _chk(lib().cgfdm_source_scales(float(DH), None, None, 0, 0, 0.0, 0.0, ctypes.byref(cv), ctypes.byref(cs)))
+    else:
+        t = np.ascontiguousarray(table, np.float32)
+        x = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        if len(x) != t.shape[1]:
+            raise ValueError("xyz and table disagree on the number of nodes")
+        _chk(lib().cgfdm_source_scales(float(DH), _p(x), _p(t), t.shape[1], t.shape[0], float(dt), float(mu_max),
+                                       ctypes.byref(cv), ctypes.byref(cs)))
+    return cv.value, cs.value
+
+
 class ServerConfig(ctypes.Structure):
     """Mirror of struct cgfdm_server_config."""
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),

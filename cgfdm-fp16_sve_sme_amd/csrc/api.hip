@@ -432,6 +432,20 @@ long long cgfdm_estimate_bytes(const cgfdm_desc* desc) {
     return estimate(*desc);
 }
 
+int cgfdm_source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, float dt,
+                        double mu_max, double* Cv, double* Cs) {
+    if (!Cv || !Cs || !(DH > 0) || (rate && (!xyz || npts < 1 || nt < 1 || !(dt > 0) || !(mu_max > 0)))) {
+        set_err("source_scales needs DH > 0 and, with a table, its nodes, npts, nt >= 1, dt > 0 and mu_max > 0");
+        return CGFDM_EINVAL;
+    }
+    source_scales(DH, xyz, rate, npts, nt, dt, mu_max, *Cv, *Cs);
+    if (!(*Cv > 0) || !std::isfinite(*Cv)) {
+        set_err("source_scales: the table gives no finite scale");
+        return CGFDM_EINVAL;
+    }
+    return CGFDM_OK;
+}
+
 int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) {
     if (!out) { set_err("null output handle"); return CGFDM_EINVAL; }
     *out = nullptr;

@@ -14,13 +14,17 @@
 //   {OUT}/coord?_*, Vs_*, Vp_*, rho_* model planes (data_io.cpp:497-526)
 //
 // Usage: cgfdm_run [--params params.json] [--stations station.json] [--precision fp32|fp16|bf16]
-//                  [--device N] [--model-only] [--chi 2lambda+mu|lambda+2mu]
+//                  [--device N] [--model-only] [--chi 2lambda+mu|lambda+2mu] [--scales reference|auto]
 // --model-only writes the model planes and stops before any device call.
 // --chi selects the coefficient of the free-surface matrices: the reference's 2*lambda + mu
 // (contravariant.cpp:266, the default, for parity) or lambda + 2*mu, the traction-free condition;
 // the run prints which one it uses.
+// --scales (16-bit storage) keeps the reference's fixed scale factors (the default) or chooses them
+// from the source (cgfdm_source_scales, with the medium's largest shear modulus), so a strong
+// source stays inside the FP16 range.
 // Exit status follows the reference's abort codes where a status can carry them: 100 no
 // params.json, 101 unparseable params.json (reference 1001), 110 CFL violation, 180 no source.
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -295,6 +299,7 @@ int main(int argc, char** argv) {
     int device = 0;
     bool modelOnly = false;
     int chiMode = 0;
+    bool autoScales = false;
     for (int a = 1; a < argc; ++a) {
         std::string s = argv[a];
         auto next = [&]() -> std::string { return a + 1 < argc ? argv[++a] : ""; };
@@ -308,6 +313,12 @@ int main(int argc, char** argv) {
             if (v == "2lambda+mu") chiMode = 0;
             else if (v == "lambda+2mu") chiMode = 1;
             else { std::fprintf(stderr, "--chi takes 2lambda+mu or lambda+2mu\n"); return 2; }
+        }
+        else if (s == "--scales") {
+            const std::string v = next();
+            if (v == "reference") autoScales = false;
+            else if (v == "auto") autoScales = true;
+            else { std::fprintf(stderr, "--scales takes reference or auto\n"); return 2; }
         }
         else { std::fprintf(stderr, "unknown argument %s\n", s.c_str()); return 2; }
     }
@@ -447,33 +458,48 @@ int main(int argc, char** argv) {
     }
     std::printf("free surface: chi = %s\n", chiMode ? "lambda + 2 mu (traction-free, --chi lambda+2mu)"
                                                    : "2 lambda + mu (the reference's, contravariant.cpp:266)");
+    int rc = 0;
+    // init_MultiSource (multiSource.cpp:824-922): the finite-fault file mapped onto the grid (host
+    // work, before the domain exists, so the scales can follow the source)
+    long long nsrc = 0;
+    int snt = 0;
+    float sdt = 0.f;
+    std::vector<int> sxyz;
+    std::vector<float> srate;
+    if (P.useMultiSource) {
+        const std::string path = P.sourceDir + "/" + P.sourceFile;
+        const double* tp = P.gauss_hill ? terrain.data() : nullptr;
+        rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, nullptr, nullptr, 0,
+                                    &nsrc, &snt, &sdt);
+        sxyz.resize((size_t)std::max<long long>(nsrc, 1) * 3);
+        srate.resize((size_t)std::max<long long>(nsrc, 1) * snt * 6);
+        if (!rc)
+            rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, sxyz.data(),
+                                        srate.data(), nsrc, &nsrc, &snt, &sdt);
+        if (rc) { std::fprintf(stderr, "source file: %s\n", cgfdm_last_error()); return 1; }
+        std::printf("finite fault %s: %lld grid nodes, nt = %d, dt = %g\n", path.c_str(), nsrc, snt, sdt);
+    }
+    if (autoScales && d.precision != CGFDM_FP32) {
+        double mu = 0.0;
+        for (size_t q = 0; q < nin; ++q) mu = std::max(mu, (double)rho[q] * vs[q] * vs[q]);
+        const bool table = P.useMultiSource && nsrc > 0;
+        if ((rc = cgfdm_source_scales(d.DH, table ? sxyz.data() : nullptr, table ? srate.data() : nullptr,
+                                      table ? nsrc : 0, table ? snt : 0, table ? sdt : 0.f, mu, &d.Cv, &d.Cs))) {
+            std::fprintf(stderr, "scales: %s\n", cgfdm_last_error());
+            return 1;
+        }
+        std::printf("scales from the source: Cv = %.6g, Cs = %.6g\n", d.Cv, d.Cs);
+    }
     cgfdm_domain* h = nullptr;
-    int rc = cgfdm_domain_create(&d, device, &h);
+    rc = cgfdm_domain_create(&d, device, &h);
     if (rc) { std::fprintf(stderr, "create: %s\n", cgfdm_last_error()); return 1; }
     auto guard = std::unique_ptr<cgfdm_domain, int (*)(cgfdm_domain*)>(h, cgfdm_domain_destroy);
     rc = cgfdm_set_model(h, P.gauss_hill ? terrain.data() : nullptr, vs.data(), vp.data(), rho.data());
     if (rc == CGFDM_ECFL) { std::fprintf(stderr, "The parameters can't afford the CFL condition! %s\n", cgfdm_last_error()); return 110; }
     if (rc) { std::fprintf(stderr, "set_model: %s\n", cgfdm_last_error()); return 1; }
-    if (P.useMultiSource) {
-        // init_MultiSource (multiSource.cpp:824-922): the finite-fault file mapped onto the grid
-        const std::string path = P.sourceDir + "/" + P.sourceFile;
-        const double* tp = P.gauss_hill ? terrain.data() : nullptr;
-        long long n = 0;
-        int snt = 0;
-        float sdt = 0.f;
-        rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, nullptr, nullptr, 0, &n,
-                                    &snt, &sdt);
-        std::vector<int> sxyz((size_t)std::max<long long>(n, 1) * 3);
-        std::vector<float> srate((size_t)std::max<long long>(n, 1) * snt * 6);
-        if (!rc)
-            rc = cgfdm_finite_fault_map(path.c_str(), &d, tp, P.centerLongitude, P.centerLatitude, sxyz.data(),
-                                        srate.data(), n, &n, &snt, &sdt);
-        if (rc) { std::fprintf(stderr, "source file: %s\n", cgfdm_last_error()); return 1; }
-        std::printf("finite fault %s: %lld grid nodes, nt = %d, dt = %g\n", path.c_str(), n, snt, sdt);
-        if (n > 0 && (rc = cgfdm_set_source_table(h, sxyz.data(), srate.data(), n, snt, sdt))) {
-            std::fprintf(stderr, "source: %s\n", cgfdm_last_error());
-            return 1;
-        }
+    if (P.useMultiSource && nsrc > 0 && (rc = cgfdm_set_source_table(h, sxyz.data(), srate.data(), nsrc, snt, sdt))) {
+        std::fprintf(stderr, "source: %s\n", cgfdm_last_error());
+        return 1;
     }
     if (P.useSingleSource) {
         rc = cgfdm_set_source_point(h, P.sourceX, P.sourceY, P.sourceZ, P.rickerfc);

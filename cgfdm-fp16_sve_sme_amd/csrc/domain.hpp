@@ -9,6 +9,8 @@
 //    the server's, and set-up calls only enqueue work (no host synchronisation), so admitting a
 //    request never stalls the simulations already running.
 #pragma once
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -99,6 +101,72 @@ struct cgfdm_domain {
 };
 
 namespace cg {
+
+// Reference scale factors of 16-bit storage (macro.h:65-73): Cv/Cs = 5e9 fixes the stored medium
+// (rho*Cs/Cv), kept by every choice below.
+constexpr double kRefCv = 5.0e3, kRefCs = 1.0e-6, kRefCrho = 1.0e6;
+// Targets of the scaled peak particle velocity and stress that source_scales aims at: the
+// near-field estimates below exceed the measured peaks (by ~1.6x on the throughput workload),
+// and 1e4 leaves a further 6x below the FP16 maximum (65504).
+constexpr double kVelTarget = 1.0e4, kStressTarget = 1.0e4;
+
+// Scale factors (Cv, Cs) of 16-bit storage chosen from the source. Point source (rate == null):
+// the reference's (5e3, 1e-6), calibrated for its fixed 1e16 N m Ricker source in 100 m cells,
+// times (DH/100)^2 (the near-field velocity of a fixed moment rate falls as 1/DH^2). Moment-rate
+// table (nt x npts x 6, tensor * area * slip rate, as cgfdm_set_source_table takes it, at grid
+// nodes xyz): the peak particle velocity is estimated as the largest, over nodes p and radii
+// R = DH, 2 DH, ..., 16 DH, of 2 * sum_{|q - p| <= R} max_t|rate_q| / (4 pi R^2). At R = DH this
+// is the near field of one node's moment rate M' = mu * rate one cell away, M'/(4 pi mu DH^2)
+// (mu cancels); at larger R it is the field of a patch of nodes slipping together, which tends to
+// the slip rate / 2 of a planar fault (each node holding area * slip rate). The peak stress is
+// estimated from the static near field, mu_max * max_t|sum rate dt| / (4 pi DH^3). Cv is the
+// larger value that keeps both scaled estimates below their targets, Cs = Cv / 5e9.
+inline void source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, double dt,
+                          double mu_max, double& Cv, double& Cs) {
+    const double ratio = kRefCv / kRefCs;
+    if (!rate) {
+        Cv = kRefCv * (DH / 100.0) * (DH / 100.0);
+        Cs = Cv / ratio;
+        return;
+    }
+    std::vector<double> peak((size_t)npts, 0.0);  // max over time and components of |rate| per node
+    double mmax = 0.0;
+    for (long long p = 0; p < npts; ++p)
+        for (int c = 0; c < 6; ++c) {
+            double acc = 0.0;
+            for (int t = 0; t < nt; ++t) {
+                const double r = rate[((long long)t * npts + p) * 6 + c];
+                peak[p] = std::max(peak[p], std::fabs(r));
+                acc += r * dt;
+                mmax = std::max(mmax, std::fabs(acc));
+            }
+        }
+    const double pi4 = 4.0 * 3.14159265358979323846;
+    double v = 0.0;
+    // patch sums around every node (every stride-th node as a centre above 20000 nodes)
+    const long long stride = npts > 20000 ? (npts + 19999) / 20000 : 1;
+    for (long long p = 0; p < npts; p += stride) {
+        double sum[5] = {0, 0, 0, 0, 0};
+        for (long long q = 0; q < npts; ++q) {
+            const double dx = xyz[3 * q] - xyz[3 * p], dy = xyz[3 * q + 1] - xyz[3 * p + 1],
+                         dz = xyz[3 * q + 2] - xyz[3 * p + 2];
+            const double d2 = dx * dx + dy * dy + dz * dz;  // in cells squared
+            for (int r = 0; r < 5; ++r)
+                if (d2 <= (double)(1 << r) * (1 << r)) sum[r] += peak[q];
+        }
+        for (int r = 0; r < 5; ++r) {
+            const double R = DH * (1 << r);
+            v = std::max(v, (r == 0 ? 1.0 : 2.0) * sum[r] / (pi4 * R * R));
+        }
+    }
+    const double T = mu_max * mmax / (pi4 * DH * DH * DH);
+    Cv = kRefCv;
+    if (v > 0.0 || T > 0.0) {
+        Cv = v > 0.0 ? kVelTarget / v : 1e300;
+        if (T > 0.0) Cv = std::min(Cv, ratio * kStressTarget / T);
+    }
+    Cs = Cv / ratio;
+}
 
 // Set-up entry points shared by the C ABI and the server.
 int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device);

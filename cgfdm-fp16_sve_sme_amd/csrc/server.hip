@@ -23,6 +23,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -159,6 +160,9 @@ void footprint(Req& r) {
 
 }  // namespace
 
+// Least-work admission passes a waiting request over for at most this long.
+constexpr std::chrono::seconds kAdmitAge{60};
+
 struct cgfdm_server {
     cgfdm_server_config cfg{};
     int cfgTile = 1;  // bulk tile shape of the batch kernels (0: 64x8, 1: 64x4)
@@ -191,6 +195,7 @@ struct cgfdm_server {
     std::thread worker;
 
     int setup_resident(Resident& R);
+    int capture_graph();
     void fail_all(int code, const std::string& msg);
     void loop();
 };
@@ -313,26 +318,63 @@ void cgfdm_server::fail_all(int code, const std::string& msg) {
     cvDone.notify_all();
 }
 
+// Captures steps_per_launch whole batch steps (every kernel over the device tables, the boundary
+// kernels forked onto the side stream and joined back) into a graph replayed by every launch.
+int cgfdm_server::capture_graph() {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { set_err("capture: %s", hipGetErrorString(e)); return CGFDM_EHIP; }
+    int rc = 0;
+    const StageFork fk = fork();
+    for (int s = 0; s < cfg.steps_per_launch && rc == 0; ++s)
+        rc = batch_step(cfg.precision, cfgTile, b, grid, stream, &fk);
+    e = hipStreamEndCapture(stream, &g);
+    if (rc || e != hipSuccess) {
+        set_err("graph capture failed: %s", hipGetErrorString(e));
+        if (g) (void)hipGraphDestroy(g);
+        return CGFDM_EHIP;
+    }
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { set_err("graph instantiate: %s", hipGetErrorString(e)); return CGFDM_EHIP; }
+    return CGFDM_OK;
+}
+
 void cgfdm_server::loop() {
     if (hipSetDevice(cfg.device) != hipSuccess) { fail_all(CGFDM_EHIP, "hipSetDevice failed in the worker"); return; }
     std::vector<Resident> finishing[2];  // retired at launch n, completed when ev[n & 1] has passed
     long long n = 0;
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        // admit in arrival order, or the least work first
+        // admit in arrival order (the earliest submitted request, which waits for room if it does
+        // not fit), or the least work first: requests waiting longer than kAdmitAge go first,
+        // oldest first (a bound on how long a large request can be passed over), then the
+        // rest by increasing NX*NY*NZ*NT; the first of them that fits the arenas is admitted
         while (!pending.empty() && !freeSlots.empty()) {
-            size_t pick = 0;
+            std::vector<size_t> order(cfg.admit_order == 1 ? pending.size() : 1);
+            for (size_t q = 0; q < order.size(); ++q) order[q] = q;
             if (cfg.admit_order == 1) {
-                auto work = [](const Req& r) {
-                    return (long long)r.desc.NX * r.desc.NY * r.desc.NZ * (long long)r.desc.NT;
+                const auto now = Clock::now();
+                auto aged = [&](size_t q) { return now - pending[q]->tSubmit > kAdmitAge; };
+                auto work = [&](size_t q) {
+                    const cgfdm_desc& d = pending[q]->desc;
+                    return (long long)d.NX * d.NY * d.NZ * (long long)d.NT;
                 };
-                for (size_t q = 1; q < pending.size(); ++q)
-                    if (work(*pending[q]) < work(*pending[pick])) pick = q;
+                std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+                    const bool ax = aged(x), ay = aged(y);
+                    if (ax != ay) return ax;
+                    return ax ? false : work(x) < work(y);  // aged ones keep arrival order
+                });
             }
-            Req& head = *pending[pick];
-            size_t doff, hoff;
-            if (!devArena.alloc(head.devBytes, doff)) break;
-            if (!hostArena.alloc(head.hostBytes, hoff)) { devArena.release(doff, head.devBytes); break; }
+            size_t pick = pending.size(), doff = 0, hoff = 0;
+            for (size_t q : order) {
+                const Req& r = *pending[q];
+                if (!devArena.alloc(r.devBytes, doff)) continue;
+                if (!hostArena.alloc(r.hostBytes, hoff)) { devArena.release(doff, r.devBytes); continue; }
+                pick = q;
+                break;
+            }
+            if (pick == pending.size()) break;
             Resident R;
             R.req = std::move(pending[pick]);
             pending.erase(pending.begin() + (long)pick);
@@ -492,24 +534,32 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         if ((e = hipEventCreateWithFlags(&v, hipEventDisableTiming)) != hipSuccess) { set_err("events"); return undo(CGFDM_EHIP); }
     if (batch_grid(c->precision, S->cfgTile, S->grid)) { set_err("occupancy query failed"); return undo(CGFDM_EHIP); }
     if ((e = hipStreamSynchronize(S->stream)) != hipSuccess) { set_err("init sync: %s", hipGetErrorString(e)); return undo(CGFDM_EHIP); }
-    if (c->use_graph) {
-        hipGraph_t g = nullptr;
-        if ((e = hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) { set_err("capture: %s", hipGetErrorString(e)); return undo(CGFDM_EHIP); }
-        int rc = 0;
-        const StageFork fk = S->fork();
-        for (int s = 0; s < c->steps_per_launch && rc == 0; ++s)
-            rc = batch_step(c->precision, S->cfgTile, S->b, S->grid, S->stream, &fk);
-        e = hipStreamEndCapture(S->stream, &g);
-        if (rc || e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); if (g) (void)hipGraphDestroy(g); return undo(CGFDM_EHIP); }
-        e = hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) { set_err("graph instantiate: %s", hipGetErrorString(e)); return undo(CGFDM_EHIP); }
-    }
     for (int s = c->max_domains - 1; s >= 0; --s) S->freeSlots.push_back(s);
     S->t0 = Clock::now();
-    cgfdm_server* raw = S.release();
-    raw->worker = std::thread([raw]() { raw->loop(); });
-    *out = raw;
+    // The worker thread captures the step graph (use_graph) before it serves anything, so the graph
+    // is captured, instantiated and replayed on the one thread that launches work on the stream;
+    // creation waits for that and reports its failure.
+    auto ready = std::make_shared<std::promise<std::pair<int, std::string>>>();
+    auto started = ready->get_future();
+    cgfdm_server* raw = S.get();
+    S->worker = std::thread([raw, ready]() {
+        int rc = 0;
+        if (hipSetDevice(raw->cfg.device) != hipSuccess) {
+            set_err("hipSetDevice failed in the worker");
+            rc = CGFDM_EHIP;
+        }
+        if (rc == 0 && raw->cfg.use_graph) rc = raw->capture_graph();
+        ready->set_value({rc, rc ? g_err : std::string()});
+        if (rc == 0) raw->loop();
+    });
+    const auto res = started.get();
+    if (res.first) {
+        S->worker.join();
+        if (S->exec) { (void)hipGraphExecDestroy(S->exec); S->exec = nullptr; }
+        set_err("%s", res.second.c_str());
+        return undo(res.first);
+    }
+    *out = S.release();
     return CGFDM_OK;
 }
 
@@ -555,6 +605,27 @@ int cgfdm_server_submit(cgfdm_server* s, const cgfdm_request* q, long long* id) 
     } else {
         set_err("source must be 1 (point) or 2 (table)");
         return CGFDM_EINVAL;
+    }
+    if (d.precision != CGFDM_FP32 && !(d.Cv > 0) && !(d.Cs > 0)) {
+        // per-request scales from the source (cgfdm_source_scales), with the largest shear
+        // modulus of the request's medium
+        double mu = 0.0;
+        if (r->nl > 0) {
+            for (int l = 0; l < r->nl; ++l)
+                mu = std::max(mu, (double)r->layers[3 * r->nl + l] * r->layers[r->nl + l] * r->layers[r->nl + l]);
+            mu = std::max(mu, 2670.0 * 3464.0 * 3464.0);  // points outside every layer keep the default medium
+        } else {
+            for (size_t t = 0; t < nin; ++t) mu = std::max(mu, (double)r->rho[t] * r->vs[t] * r->vs[t]);
+        }
+        double Cv = 0.0, Cs = 0.0;
+        if (q->source == 2)
+            source_scales(d.DH, r->tabXyz.data(), r->tabRate.data(), r->tabNpts, r->tabNt, r->tabDt, mu, Cv, Cs);
+        else
+            source_scales(d.DH, nullptr, nullptr, 0, 0, 0.0, mu, Cv, Cs);
+        if (Cv > 0 && std::isfinite(Cv)) {
+            r->desc.Cv = Cv;
+            r->desc.Cs = Cs;
+        }
     }
     if (q->nst < 0 || (q->nst > 0 && !q->st_xyz)) { set_err("bad station list"); return CGFDM_EINVAL; }
     r->st.assign(q->st_xyz, q->st_xyz + 3 * (size_t)q->nst);

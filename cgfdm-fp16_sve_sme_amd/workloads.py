@@ -215,3 +215,193 @@ def serving_stream(n, seed, rate_hz=20.0, edge_xy=(101, 301), edge_z=(51, 151), 
                         "precision": precision, "layers": lay,
                         "table": (np.array([[X, Y, Z]], np.int32), table, DT), "stations": st}))
     return out
+
+
+# ------------------------------------------------------------------ kinematic finite-fault rupture
+WGS84_A = 6378137.0
+WGS84_F = 1.0 / 298.257223563
+
+
+def fractal_terrain(NX, NY, DH, hurst=0.8, relief=2000.0, min_wavelength=1000.0, seed=0):
+    """Seeded self-affine (fractional Brownian) surface on the padded (NY+6, NX+6) grid: spectral
+    synthesis with amplitude |k|^-(hurst+1) (power |k|^-(2 hurst + 2)) and random phases, a
+    Gaussian roll-off exp(-(|k| min_wavelength)^2) that removes wavelengths below min_wavelength
+    (so the terrain-following grid stays smooth on the cell scale), shifted and scaled to heights
+    in [0, relief] metres."""
+    rng = np.random.default_rng(seed)
+    n1, n0 = NY + 6, NX + 6
+    kx = np.fft.fftfreq(n0, d=DH)
+    ky = np.fft.fftfreq(n1, d=DH)
+    K = np.hypot(*np.meshgrid(kx, ky))
+    amp = np.zeros_like(K)
+    nz = K > 0
+    amp[nz] = K[nz] ** -(hurst + 1.0) * np.exp(-(K[nz] * min_wavelength) ** 2)
+    spec = amp * (rng.standard_normal(K.shape) + 1j * rng.standard_normal(K.shape))
+    h = np.real(np.fft.ifft2(spec))
+    h -= h.min()
+    return h * (relief / max(h.max(), 1e-30))
+
+
+def von_karman_slip(n_strike, n_dip, d_strike, d_dip, corr_strike, corr_dip, hurst, mean, seed, taper=0.15):
+    """Seeded random slip (m) on an n_dip x n_strike subfault grid with the von Karman spectrum
+    P(k) = 1 / (1 + (a_s k_s)^2 + (a_d k_d)^2)^(hurst + 1) (Mai and Beroza 2002, k in rad/m, a the
+    correlation lengths), made non-negative, tapered to zero over `taper` of the fault at all four
+    edges (a buried rupture) with a cosine, and scaled to the given mean."""
+    rng = np.random.default_rng(seed)
+    ks = 2 * math.pi * np.fft.fftfreq(n_strike, d=d_strike)
+    kd = 2 * math.pi * np.fft.fftfreq(n_dip, d=d_dip)
+    KS, KD = np.meshgrid(ks, kd)
+    P = 1.0 / (1.0 + (corr_strike * KS) ** 2 + (corr_dip * KD) ** 2) ** (hurst + 1.0)
+    phase = np.exp(2j * math.pi * rng.random(P.shape))
+    f = np.real(np.fft.ifft2(np.sqrt(P) * phase))
+    f -= f.min()
+
+    def edge(n):
+        w = np.ones(n)
+        m = max(1, int(round(taper * n)))
+        ramp = 0.5 * (1 - np.cos(math.pi * (np.arange(m) + 0.5) / m))
+        w[:m] = ramp
+        w[n - m:] = ramp[::-1]
+        return w
+
+    f *= edge(n_dip)[:, None] * edge(n_strike)[None, :]
+    return f * (mean / f.mean())
+
+
+def triangle_slip_rate(t, t0, rise, slip):
+    """Slip rate (m/s) of a triangle source-time function starting at t0 with rise time `rise`
+    whose integral is `slip`; arrays broadcast (t along the last axis)."""
+    u = (t - t0) / rise
+    return np.where((u >= 0) & (u <= 1), 2.0 * slip / rise * (1.0 - np.abs(2.0 * u - 1.0)), 0.0)
+
+
+def geodesic_direct(lon0, lat0, azimuth, distance):
+    """End point (degrees) of the WGS84 geodesic from (lon0, lat0) (degrees) along azimuth
+    (radians, clockwise from north) for `distance` metres: Vincenty's direct solution (Survey
+    Review 23(176), 1975). The inverse of the azimuthal equidistant projection centred on
+    (lon0, lat0): x = s sin(az), y = s cos(az)."""
+    a, f = WGS84_A, WGS84_F
+    b = a * (1 - f)
+    az = np.asarray(azimuth, np.float64)
+    s = np.asarray(distance, np.float64)
+    phi1 = math.radians(lat0)
+    tanU1 = (1 - f) * math.tan(phi1)
+    cosU1 = 1 / math.sqrt(1 + tanU1 * tanU1)
+    sinU1 = tanU1 * cosU1
+    sinA1, cosA1 = np.sin(az), np.cos(az)
+    sig1 = np.arctan2(tanU1, cosA1)
+    sinA = cosU1 * sinA1
+    cos2A = 1 - sinA * sinA
+    u2 = cos2A * (a * a - b * b) / (b * b)
+    A = 1 + u2 / 16384 * (4096 + u2 * (-768 + u2 * (320 - 175 * u2)))
+    B = u2 / 1024 * (256 + u2 * (-128 + u2 * (74 - 47 * u2)))
+    sig = s / (b * A)
+    for _ in range(100):
+        c2sm = np.cos(2 * sig1 + sig)
+        sinS, cosS = np.sin(sig), np.cos(sig)
+        dsig = B * sinS * (c2sm + B / 4 * (cosS * (-1 + 2 * c2sm * c2sm)
+                                             - B / 6 * c2sm * (-3 + 4 * sinS * sinS) * (-3 + 4 * c2sm * c2sm)))
+        new = s / (b * A) + dsig
+        done = np.max(np.abs(new - sig)) < 1e-14
+        sig = new
+        if done:
+            break
+    c2sm = np.cos(2 * sig1 + sig)
+    sinS, cosS = np.sin(sig), np.cos(sig)
+    t = sinU1 * sinS - cosU1 * cosS * cosA1
+    phi2 = np.arctan2(sinU1 * cosS + cosU1 * sinS * cosA1, (1 - f) * np.sqrt(sinA * sinA + t * t))
+    lam = np.arctan2(sinS * sinA1, cosU1 * cosS - sinU1 * sinS * cosA1)
+    C = f / 16 * cos2A * (4 + f * (4 - 3 * cos2A))
+    L = lam - (1 - C) * f * sinA * (sig + C * sinS * (c2sm + C * cosS * (-1 + 2 * c2sm * c2sm)))
+    return lon0 + np.degrees(L), np.degrees(phi2)
+
+
+def aeqd_inverse(lon0, lat0, x, y):
+    """(lon, lat) in degrees of the point at (x east, y north) metres in the azimuthal
+    equidistant projection on WGS84 centred on (lon0, lat0)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    return geodesic_direct(lon0, lat0, np.arctan2(x, y), np.hypot(x, y))
+
+
+def write_fault_file(path, lon, lat, z, area, strike, dip, rake, rate, dt):
+    """A finite-fault source file in the reference's layout (the output of
+    constructFiniteFaultModel/readUSGS.py, read by multiSource.cpp:14-37 and
+    cgfdm_finite_fault_map): int32 npts, int32 nt, float32 dt, then per point float32 lon, lat
+    (degrees), z (m), area (m^2), strike, dip (degrees), rake[nt] (degrees), rate[nt] (m/s)."""
+    rake = np.asarray(rake, np.float32)
+    rate = np.asarray(rate, np.float32)
+    npts, nt = rate.shape
+    head = np.stack([lon, lat, z, area, strike, dip], axis=1).astype(np.float32)
+    rec = np.concatenate([head, np.broadcast_to(rake, (npts, nt)), rate], axis=1).astype(np.float32)
+    with open(path, "wb") as f:
+        np.array([npts, nt], np.int32).tofile(f)
+        np.array([dt], np.float32).tofile(f)
+        rec.tofile(f)
+
+
+def finite_fault_rupture(lon0=100.0, lat0=30.0, n_strike=80, n_dip=20, length=60e3, width=20e3, strike=90.0,
+                         dip=60.0, rake=90.0, mean_slip=2.0, hurst=0.75, vr=2800.0, rise=(1.0, 3.0), dt=0.1,
+                         y_top=5e3, z_top=-500.0, seed=11):
+    """The kinematic rupture of the finite-fault workload: an n_dip x n_strike grid of subfaults on
+    a length x width plane (strike 90: along x, east; dipping `dip` degrees to the south, top edge
+    at y = y_top and depth -z_top), von Karman slip (correlation lengths 2 km + L/3 along strike
+    and 1 km + W/3 down dip, Mai and Beroza 2002; Hurst `hurst`; mean `mean_slip` m), rupture
+    times from a front at vr m/s out of a seeded random hypocentre, triangle slip-rate functions
+    whose rise time grows with the square root of the slip from rise[0] to rise[1] s, constant
+    rake. Returns the fields of write_fault_file plus the subfault x, y (m) and the hypocentre."""
+    rng = np.random.default_rng(seed)
+    ds, dd = length / n_strike, width / n_dip
+    slip = von_karman_slip(n_strike, n_dip, ds, dd, 2e3 + length / 3, 1e3 + width / 3, hurst, mean_slip,
+                           seed=seed + 1)
+    s = (np.arange(n_strike) + 0.5) * ds                    # along strike from the west end
+    w = (np.arange(n_dip) + 0.5) * dd                       # down dip from the top edge
+    S, Wd = np.meshgrid(s, w)
+    d = math.radians(dip)
+    x = S - length / 2
+    y = y_top - Wd * math.cos(d)
+    z = z_top - Wd * math.sin(d)
+    hs, hw = rng.uniform(0.15, 0.85) * length, rng.uniform(0.5, 0.85) * width
+    t0 = np.hypot(S - hs, Wd - hw) / vr
+    tr = rise[0] + (rise[1] - rise[0]) * np.sqrt(slip / slip.max())
+    nt = int(math.ceil((t0.max() + tr.max()) / dt)) + 3
+    t = np.arange(nt) * dt
+    rate = triangle_slip_rate(t[None, :], t0.reshape(-1, 1), tr.reshape(-1, 1), slip.reshape(-1, 1))
+    lon, lat = aeqd_inverse(lon0, lat0, x.ravel(), y.ravel())
+    npts = x.size
+    return {"lon": lon, "lat": lat, "z": z.ravel(), "area": np.full(npts, ds * dd), "strike": np.full(npts, strike),
+            "dip": np.full(npts, dip), "rake": np.full((npts, nt), rake), "rate": rate, "dt": dt,
+            "x": x.ravel(), "y": y.ravel(), "slip": slip.ravel(), "t0": t0.ravel(), "rise": tr.ravel(),
+            "hypocentre": (hs - length / 2, y_top - hw * math.cos(d), z_top - hw * math.sin(d))}
+
+
+def finite_fault_workload(DH=100.0, seed=2031, lon0=100.0, lat0=30.0, build=True):
+    """The kinematic finite-fault workload on a 120 km x 120 km x 40 km box: 1201 x 1201 x 401
+    points at DH = 100 m (241 x 241 x 81 at 500 m), DT = 8 ms * DH / 100 m (inside the
+    reference's CFL limit 1.34 Hmin / Vpmax on this model) and 4000 steps at 100 m, i.e. 32 s of
+    simulated time, fractal terrain (Hurst 0.8, 2 km relief, wavelengths
+    below 10 DH removed), the 3-layer medium with a 5 % random perturbation of 1 km correlation,
+    the rupture of finite_fault_rupture (80 x 20 subfaults, 60 km x 20 km, strike 90, dip 60, von
+    Karman slip of mean 2 m, 2.8 km/s rupture front, triangle slip rates of 1-3 s rise time) with
+    the projection centred on (lon0, lat0) at the grid centre, and 20 x 20 = 400 surface stations.
+    With build=False the terrain and medium are left out (None)."""
+    NX = NY = int(round(120e3 / DH)) + 1
+    NZ = int(round(40e3 / DH)) + 1
+    DT = 0.008 * DH / 100.0
+    NT = int(round(32.0 / DT))
+    depth = NZ * DH
+    terrain = medium = None
+    if build:
+        terrain = fractal_terrain(NX, NY, DH, hurst=0.8, relief=2000.0, min_wavelength=10 * DH, seed=seed)
+        medium = build_medium(NX, NY, NZ, DH, terrain=terrain, depth=depth, pert=0.05, corr_m=1000.0, seed=seed + 1)
+    return {"grid": (NX, NY, NZ), "DH": DH, "DT": DT, "NT": NT, "depth": depth, "nPML": 12,
+            "lon0": lon0, "lat0": lat0, "terrain": terrain, "medium": medium,
+            "rupture": finite_fault_rupture(lon0, lat0, seed=seed + 2),
+            "stations": surface_stations(NX, NY, NZ, n=20)}
+
+
+def write_rupture(path, rupture):
+    """Writes the rupture of finite_fault_rupture as a reference-layout source file."""
+    r = rupture
+    write_fault_file(path, r["lon"], r["lat"], r["z"], r["area"], r["strike"], r["dip"], r["rake"], r["rate"],
+                     r["dt"])

@@ -85,6 +85,23 @@ long long cgfdm_domain_bytes(const cgfdm_domain* h);
 /* Bytes cgfdm_domain_create would allocate for this descriptor (no allocation). */
 long long cgfdm_estimate_bytes(const cgfdm_desc* desc);
 
+/* Scale factors of 16-bit storage chosen from the source, for cgfdm_desc.Cv/Cs (Crho unchanged),
+ * so that a strong source stays inside the FP16 range where the reference's fixed scales
+ * (macro.h:65-73) overflow. Cv/Cs stays at the reference's 5e9, so the stored medium is unchanged.
+ *  rate == NULL: the point source (fixed 1e16 N*m Ricker, singleSource.cpp:29-56): the reference's
+ *      (5e3, 1e-6) times (DH/100 m)^2.
+ *  rate = a moment-rate table as cgfdm_set_source_table takes it (nt x npts x 6, tensor * area *
+ *      slip rate, at the global interior indices xyz[3*npts]) sampled every dt s, mu_max the
+ *      largest shear modulus (Pa) of the medium: the peak velocity is estimated as the largest,
+ *      over nodes p and radii R = DH .. 16 DH, of (1 at R = DH, else 2) * sum over the nodes
+ *      within R of max_t|rate| / (4 pi R^2) (the near field of one node's moment rate; the
+ *      field of a patch of nodes slipping together, which tends to half the slip rate of a
+ *      planar fault), the peak stress as mu_max * max_t|sum rate dt| / (4 pi DH^3);
+ *      Cv = min(1e4 / velocity, 5e9 * 1e4 / stress).
+ * No counterpart in the reference, whose scales are compile-time constants. */
+int cgfdm_source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, float dt,
+                        double mu_max, double* Cv, double* Cs);
+
 /* Uploads the model and builds the packed metric/Jacobian/medium array in FP64 on the GPU,
  * the free-surface rDZ matrices and the PML profiles.
  *  terrain : height (m) of the surface on the padded (NY+6)x(NX+6) grid, x fastest;
@@ -241,13 +258,19 @@ typedef struct cgfdm_server_config {
     int steps_per_launch;   /* whole time steps per batch launch, 1..64                   */
     int use_graph;          /* 1: replay a captured HIP graph; 0: launch directly         */
     int admit_order;        /* which waiting request a free slot takes: 0 the earliest
-                               submitted, 1 the least work (NX*NY*NZ*NT) — a lower median
-                               latency when more requests wait than slots                 */
+                               submitted (it waits for room in the arena, holding the
+                               others back); 1 the least work (NX*NY*NZ*NT) among those that
+                               fit the arena now, except that requests waiting longer than
+                               60 s go first, oldest first — a lower median latency when more
+                               requests wait than slots, and a bound on how long a large
+                               request is passed over                                       */
 } cgfdm_server_config;
 
 typedef struct cgfdm_request {
     cgfdm_desc desc;            /* grid, DH, DT, NT, nPML, ...; precision = the server's;
-                                   check_cfl must be 0                                      */
+                                   check_cfl must be 0. With 16-bit storage and Cv = Cs = 0
+                                   the server chooses the request's scales from its source
+                                   (cgfdm_source_scales, with the medium's largest mu)        */
     const double* terrain;      /* (NY+6) x (NX+6) heights when desc.use_terrain            */
     int nlayer;                 /* > 0: layered medium (as cgfdm_set_model_layered)         */
     const float* layer_depth;

@@ -8,6 +8,16 @@
                    surface, strike-slip double couple through the moment-rate table; FP16 and BF16
                    storage against the FP64 oracle (FP16 <= 1e-2 relative L2 on the seismograms
                    and PGV; BF16 <= 5e-2)
+  strong source    the reduced Gaussian-hill workload at M0 = 1e17 N m, which overflows FP16 at
+                   the reference's fixed scales: FP16 storage with scales chosen from the source
+                   (cgfdm_source_scales) against the FP64 oracle (<= 1e-2)
+  Gaussian hill    the full-size Gaussian-hill workload (401x401x201, 2000 steps, M0 = 1e17 N m):
+                   FP16 storage with scales from the source finite and within 1e-2 of FP32
+  finite fault     the kinematic finite-fault workload (workloads.finite_fault_workload): at 500 m
+                   spacing (241 x 241 x 81, 800 steps) FP32 (<= 1e-5) and FP16 with scales from the
+                   source (<= 1e-2) against the FP64 oracle's output in
+                   tests/golden/config5_reduced_fp64.npz; at full size (1201 x 1201 x 401) the first
+                   600 steps in FP16 finite and within 1e-2 of FP32
   half space       the full-size half-space workload (201x201x101, 1000 steps, 21x21 stations)
                    against the FP64 oracle's output stored in tests/golden/config1_fp64.npz
                    (<= 1e-5)
@@ -90,7 +100,7 @@ def test_bulk_tile_shapes_fp16(gpu, oracle_lib, monkeypatch, cfg):
 
 
 # ---------------------------------------------------------------- Gaussian hill, 16-bit storage
-def _hill_case(cg, prec, scales):
+def _hill_case(cg, prec, scales, M0=1e15):
     """Reduced Gaussian-hill workload: 72 x 68 x 44 at DH 100 m, hill 1500 m * (6/40) high and
     6000 m * (72/400) wide, layers at 3 and 10 km scaled by the same 44/201 depth ratio."""
     NX, NY, NZ, DH, DT, NT = 72, 68, 44, 100.0, 0.006, 360
@@ -104,7 +114,7 @@ def _hill_case(cg, prec, scales):
     X, Y, Z = NX // 2, NY // 2, NZ - 1 - 11
     vs, vp, rho = med
     mu = float(rho[Z, Y, X]) * float(vs[Z, Y, X]) ** 2
-    table = wl.double_couple_table(1e15, 0.0, 90.0, 0.0, 2.0, NT + 2, DT, mu)
+    table = wl.double_couple_table(M0, 0.0, 90.0, 0.0, 2.0, NT + 2, DT, mu)
     src = np.array([[X, Y, Z]], np.int32)
     st = wl.surface_stations(NX, NY, NZ, n=7, margin=12)
     return d, terrain, med, src, table, DT, st, NT
@@ -143,6 +153,158 @@ def test_hill_layered_table_source_low_precision_vs_fp64(gpu, hill_fp64, prec, t
     print(f"{prec} hill: velocity seismograms {ev:.2e}, stress {et:.2e}, PGV {ep:.2e}, peak {peak:+.2e}")
     assert ev <= tol and ep <= tol
     g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp16", 1e-2), ("bf16", 5e-2)])
+def test_hill_strong_source_scales_from_source_vs_fp64(gpu, oracle_lib, prec, tol):
+    """M0 = 1e17 N m, a hundred times the moment the reference's fixed scales are used with here:
+    FP16 and BF16 storage with the scales chosen from the source match the FP64 oracle."""
+    cg, po = gpu, oracle_lib
+    M0 = 1e17
+    d64, terrain, med, src, table, dt, st, NT = _hill_case(cg, "fp32", SCALES16, M0=M0)
+    o = po.Oracle(d64, po.F64)
+    o.set_model(terrain, *med)
+    o.set_source_table(src, table, dt)
+    o.set_stations(st)
+    o.step(NT)
+    so = o.stations()
+    print(f"M0 {M0:g}: oracle peak surface |v| {np.abs(so[:, :, :3]).max():.3g} m/s")
+    vs, vp, rho = med
+    Cv, Cs = cg.source_scales(100.0, src, table, dt, float((rho * vs * vs).max()))
+    d, *_ = _hill_case(cg, prec, {"Cv": Cv, "Cs": Cs, "Crho": 1e6}, M0=M0)
+    g = cg.Domain(d)
+    g.set_model(terrain, *med)
+    g.set_source_table(src, table, dt)
+    g.set_stations(st)
+    g.step(NT)
+    sg = g.stations()
+    assert np.isfinite(sg).all()
+    ev = rel_l2(sg[:, :, :3], so[:, :, :3])
+    ep = rel_l2(g.pgv(), o.pgv())
+    print(f"{prec} hill M0 1e17, Cv {Cv:.4g} Cs {Cs:.4g}: seismograms {ev:.2e}, PGV {ep:.2e} vs FP64")
+    assert ev <= tol and ep <= tol
+    g.close()
+    o.close()
+
+
+# ---------------------------------------------------------------- Gaussian hill, full size
+@pytest.mark.timeout(600)
+def test_gauss_hill_full_size_fp16_scales_from_source_vs_fp32(gpu):
+    """The Gaussian-hill workload at full size: 401 x 401 x 201, DH 100 m, hill 1500 m high and
+    6 km wide, layers Vs 2000/3000/3500 m/s below 3 and 10 km, strike-slip double couple
+    (strike 0, dip 90, rake 0) of M0 = 1e17 N m 5 km deep, 2 Hz Ricker, 2000 steps of 10 ms."""
+    cg = gpu
+    NX, NY, NZ, DH, DT, NT = 401, 401, 201, 100.0, 0.01, 2000
+    depth = NZ * DH
+    terrain = wl.gauss_hill(NX, NY, DH, h0=1500.0, width=6000.0)
+    z = models.terrain_z(terrain, NZ, depth)
+    med = models.layered_on_depth(z - terrain[3:-3, 3:-3][None], [0.0, 3.0, 10.0], [2000.0, 3000.0, 3500.0])
+    vs, vp, rho = med
+    X, Y, Z = NX // 2, NY // 2, NZ - 1 - 50
+    mu = float(rho[Z, Y, X]) * float(vs[Z, Y, X]) ** 2
+    table = wl.double_couple_table(1e17, 0.0, 90.0, 0.0, 2.0, NT + 2, DT, mu)
+    src = np.array([[X, Y, Z]], np.int32)
+    st = wl.surface_stations(NX, NY, NZ, n=21)
+    Cv, Cs = cg.source_scales(DH, src, table, DT, float((rho * vs * vs).max()))
+    out = {}
+    for prec, sc in (("fp32", {}), ("fp16", {"Cv": Cv, "Cs": Cs, "Crho": 1e6})):
+        d = cg.make_desc(NX, NY, NZ, DH=DH, DT=DT, NT=NT, nPML=12, precision=prec, use_terrain=1, depth=depth,
+                         check_cfl=1, **sc)
+        g = cg.Domain(d)
+        g.set_model(terrain, *med)
+        g.set_source_table(src, table, DT)
+        g.set_stations(st)
+        g.step(NT)
+        out[prec] = (g.stations(), g.pgv())
+        g.close()
+        print(f"Gaussian hill {prec}: {NT} steps done", flush=True)
+    s32, p32 = out["fp32"]
+    s16, p16 = out["fp16"]
+    assert np.isfinite(s16).all() and np.isfinite(p16).all()
+    e = rel_l2(s16[:, :, :3], s32[:, :, :3])
+    ep = rel_l2(p16, p32)
+    peak = float(np.abs(s32[:, :, :3]).max())
+    print(f"Gaussian hill 401x401x201, M0 1e17 (Cv {Cv:.4g}): peak surface |v| {peak:.3g} m/s, FP16 vs FP32 "
+          f"seismograms {e:.2e}, PGV {ep:.2e}")
+    assert peak > 0 and e <= 1e-2 and ep <= 1e-2
+
+
+# ---------------------------------------------------------------- kinematic finite fault
+def _fault_table(cg, w, d, tmp_path):
+    path = tmp_path / "fault.bin"
+    wl.write_rupture(path, w["rupture"])
+    return cg.finite_fault_map(path, d, w["terrain"], w["lon0"], w["lat0"])
+
+
+def _fault_scales(cg, w, xyz, table, tdt):
+    vs, vp, rho = w["medium"]
+    return cg.source_scales(w["DH"], xyz, table, tdt, float((rho.astype(np.float64) * vs * vs).max()))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("fp16", 1e-2)])
+def test_finite_fault_reduced_vs_fp64_golden(gpu, tmp_path, prec, tol):
+    cg = gpu
+    gold = np.load(os.path.join(HERE, "golden", "config5_reduced_fp64.npz"))
+    w = wl.finite_fault_workload(DH=500.0)
+    NX, NY, NZ = w["grid"]
+    NT = int(gold["nt"])
+    d32 = cg.make_desc(NX, NY, NZ, DH=w["DH"], DT=w["DT"], NT=NT, nPML=w["nPML"], use_terrain=1, depth=w["depth"])
+    xyz, table, tdt = _fault_table(cg, w, d32, tmp_path)
+    np.testing.assert_array_equal(xyz, gold["source_xyz"])  # the library maps the file as the oracle
+    sc = {}
+    if prec != "fp32":
+        Cv, Cs = _fault_scales(cg, w, xyz, table, tdt)
+        sc = {"Cv": Cv, "Cs": Cs, "Crho": 1e6}
+    d = cg.make_desc(NX, NY, NZ, DH=w["DH"], DT=w["DT"], NT=NT, nPML=w["nPML"], precision=prec, use_terrain=1,
+                     depth=w["depth"], check_cfl=1, **sc)
+    st = w["stations"]
+    sel = np.array([r * 20 + c for r in range(0, 20, 2) for c in range(0, 20, 2)])
+    np.testing.assert_array_equal(st[sel], gold["station_xyz"])
+    g = cg.Domain(d)
+    g.set_model(w["terrain"], *w["medium"])
+    g.set_source_table(xyz, table, tdt)
+    g.set_stations(st)
+    g.step(NT)
+    s = g.stations()[sel, :, :3]
+    e = rel_l2(s, gold["stations"])
+    ep = rel_l2(g.pgv(), gold["pgv"])
+    print(f"finite fault 241x241x81 {prec} {sc}: seismograms {e:.2e}, PGV {ep:.2e} vs FP64 oracle")
+    assert np.abs(gold["stations"]).max() > 0 and np.isfinite(s).all()
+    assert e <= tol and ep <= tol
+    g.close()
+
+
+@pytest.mark.timeout(600)
+def test_finite_fault_full_size_fp16_vs_fp32(gpu, tmp_path):
+    cg = gpu
+    w = wl.finite_fault_workload(DH=100.0)
+    NX, NY, NZ = w["grid"]
+    NT = 600
+    d32 = cg.make_desc(NX, NY, NZ, DH=w["DH"], DT=w["DT"], NT=NT, nPML=w["nPML"], use_terrain=1, depth=w["depth"])
+    xyz, table, tdt = _fault_table(cg, w, d32, tmp_path)
+    assert len(xyz) == 1600
+    Cv, Cs = _fault_scales(cg, w, xyz, table, tdt)
+    out = {}
+    for prec, sc in (("fp32", {}), ("fp16", {"Cv": Cv, "Cs": Cs, "Crho": 1e6})):
+        d = cg.make_desc(NX, NY, NZ, DH=w["DH"], DT=w["DT"], NT=NT, nPML=w["nPML"], precision=prec, use_terrain=1,
+                         depth=w["depth"], check_cfl=1, **sc)
+        g = cg.Domain(d)
+        g.set_model(w["terrain"], *w["medium"])
+        g.set_source_table(xyz, table, tdt)
+        g.set_stations(w["stations"])
+        g.step(NT)
+        out[prec] = (g.stations(), g.pgv())
+        g.close()
+        print(f"finite fault 1201x1201x401 {prec}: {NT} steps done", flush=True)
+    s32, p32 = out["fp32"]
+    s16, p16 = out["fp16"]
+    assert np.isfinite(s16).all() and np.isfinite(p16).all()
+    e = rel_l2(s16[:, :, :3], s32[:, :, :3])
+    ep = rel_l2(p16, p32)
+    print(f"finite fault 1201x1201x401, {NT} steps (Cv {Cv:.4g}): peak surface |v| {np.abs(s32[:, :, :3]).max():.3g} m/s, "
+          f"FP16 vs FP32 seismograms {e:.2e}, PGV {ep:.2e}")
+    assert np.abs(p32).max() > 0
+    assert e <= 1e-2 and ep <= 1e-2
 
 
 # ---------------------------------------------------------------- half space, full size

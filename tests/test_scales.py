@@ -1,0 +1,92 @@
+"""Scale factors of 16-bit storage chosen from the source (cgfdm_source_scales); a host-only entry
+point, no GPU.
+
+Pins: the point source keeps the reference's calibrated scales (5e3, 1e-6) at 100 m
+(macro.h:65-73, singleSource.cpp:29-56) and scales with DH^2; every choice keeps the reference's
+ratio Cv/Cs = 5e9 (the stored medium rho*Cs/Cv does not change); for a table, the scaled
+estimates are what the rule says they are: ten times the moment rate gives a tenth of the scales
+(the equations are linear, so the stored numbers do not change), the velocity estimate of a table
+that holds slip rate * area is independent of the medium, a planar patch of nodes slipping
+together is bounded by half its slip rate (its far-reaching coherent field) rather than by one
+node's near field, and the stress bound takes over when the accumulated moment is large and its
+rate small. The GPU tests (tests/test_gpu_configs.py)
+check that the chosen scales keep M0 = 1e17 N m sources and the finite-fault rupture finite and
+within the FP16 tolerance of the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import workloads as wl
+
+
+def test_point_source_keeps_the_reference_scales(gpu):
+    cg = gpu
+    assert cg.source_scales(100.0) == pytest.approx((5e3, 1e-6), rel=1e-12)
+    assert cg.source_scales(50.0) == pytest.approx((1250.0, 2.5e-7), rel=1e-12)
+    assert cg.source_scales(400.0) == pytest.approx((8e4, 1.6e-5), rel=1e-12)
+
+
+def test_table_scales_follow_the_peak_moment_rate(gpu):
+    cg = gpu
+    mu, dt, fc, nt = 3.0e10, 0.005, 2.0, 400
+    base = wl.double_couple_table(1e16, 0.0, 90.0, 0.0, fc, nt, dt, mu)
+    one = np.array([[50, 50, 40]], np.int32)
+    cv1, cs1 = cg.source_scales(100.0, one, base, dt, mu)
+    assert cv1 / cs1 == pytest.approx(5e9, rel=1e-12)
+    # linear in the source: ten times the moment rate, a tenth of the scales
+    cv10, _ = cg.source_scales(100.0, one, 10 * base, dt, mu)
+    assert cv10 == pytest.approx(cv1 / 10, rel=1e-6)
+    # velocity-bound here: Cv * max|rate| / (4 pi DH^2) = 1e4, and DH doubled gives Cv x 4
+    assert cv1 * np.abs(base).max() / (4 * math.pi * 1e4) == pytest.approx(1e4, rel=1e-6)
+    assert cg.source_scales(200.0, one, base, dt, mu)[0] == pytest.approx(4 * cv1, rel=1e-6)
+    # the velocity bound does not depend on the medium (the table is rate / mu at the source)
+    assert cg.source_scales(100.0, one, base, dt, 3 * mu)[0] == pytest.approx(cv1, rel=1e-6)
+    # a slow ramp in moment (40 s at a constant small rate): the stress bound
+    # (mu * moment / (4 pi DH^3)) decides instead of the velocity bound
+    step = np.zeros((nt, 1, 6), np.float32)
+    step[:, 0, 3] = 1e-3
+    cv_s, _ = cg.source_scales(100.0, one, step, 0.1, mu)
+    assert cv_s < 1e4 / (1e-3 / (4 * math.pi * 1e4))
+    M = mu * 1e-3 * 0.1 * nt
+    assert cv_s == pytest.approx(5e9 * 1e4 / (M / (4 * math.pi * 1e6)), rel=1e-5)
+
+
+def test_planar_patch_is_bounded_by_half_its_slip_rate(gpu):
+    """A square patch of nodes 5 cells apart (500 m at DH = 100 m, each with area 500^2 m^2), all
+    slipping at 2 m/s: one node's near field, area * 2 / (4 pi DH^2) = 39.8 m/s, dominates at
+    100 m spacing; at DH = 500 m (nodes one cell apart) the patch's field decides instead, within
+    10 % of half the slip rate (1 m/s)."""
+    cg = gpu
+    n = 41
+    X, Y = np.meshgrid(np.arange(n), np.arange(n))
+    slip_rate, dt, nt = 2.0, 0.01, 50
+    for dh, step in ((100.0, 5), (500.0, 1)):
+        xyz = np.stack([X.ravel() * step, Y.ravel() * step, np.full(n * n, 60)], 1).astype(np.int32)
+        area = (step * dh) ** 2
+        tab = np.zeros((nt, n * n, 6), np.float32)
+        tab[:, :, 3] = area * slip_rate  # the xy component of a vertical strike-slip, unit tensor
+        cv, _ = cg.source_scales(dh, xyz, tab, dt, 3e10)
+        v = 1e4 / cv
+        if dh == 100.0:
+            assert v == pytest.approx(area * slip_rate / (4 * math.pi * dh * dh), rel=1e-6)
+        else:
+            # one node's near field would say area * 2 / (4 pi DH^2) = 0.16 m/s, far below
+            assert v > 5 * area * slip_rate / (4 * math.pi * dh * dh)
+            assert 0.9 * slip_rate / 2 < v < 1.1 * slip_rate / 2
+
+
+def test_bad_arguments_are_rejected(gpu):
+    import ctypes
+    cg = gpu
+    cv, cs = ctypes.c_double(), ctypes.c_double()
+    assert cg.lib().cgfdm_source_scales(0.0, None, None, 0, 0, 0.0, 0.0, ctypes.byref(cv), ctypes.byref(cs)) == -1
+    t = np.ones((4, 1, 6), np.float32)
+    x = np.zeros((1, 3), np.int32)
+    for xp, npts, nt, dt, mu in ((x, 1, 4, 0.0, 1e10), (x, 1, 4, 0.01, 0.0), (x, 0, 4, 0.01, 1e10),
+                                 (None, 1, 4, 0.01, 1e10)):
+        xa = None if xp is None else xp.ctypes.data
+        assert cg.lib().cgfdm_source_scales(100.0, xa, t.ctypes.data, npts, nt, dt, mu, ctypes.byref(cv),
+                                            ctypes.byref(cs)) == -1
+    zero = np.zeros((4, 1, 6), np.float32)  # no source at all: the reference scales
+    assert cg.source_scales(100.0, x, zero, 0.01, 1e10) == pytest.approx((5e3, 1e-6))
