@@ -34,11 +34,12 @@ PEAK_HBM_GBS = 8000.0
 VALUES_PER_LUP = 192  # algorithmic values moved per grid point per full step (4 stages)
 # values the bulk kernel itself moves per point and step: the four metric components that are
 # exactly zero on terrain-following grids (xi_y, xi_z, eta_x, eta_z) are not read (4 x 4 fewer),
-# and the RK accumulator after stage 0 is rebuilt in stage 1 instead of being written and read
-# back (9 + 9 fewer); when xi_x and eta_y are the same at every point (terrain-following grids)
-# they come from one stored copy (2 x 4 fewer)
-VALUES_PER_LUP_MOVED = 158
-VALUES_PER_LUP_MOVED_UNIFORM_PAIR = 150
+# zeta_x, zeta_y, zeta_z come from per-column values (3 x 4 fewer), and the RK accumulator after
+# stage 0 is rebuilt in stage 1 instead of being written and read back (9 + 9 fewer); when xi_x
+# and eta_y are the same at every point (terrain-following grids) they come from one stored copy
+# (2 x 4 fewer)
+VALUES_PER_LUP_MOVED = 146
+VALUES_PER_LUP_MOVED_UNIFORM_PAIR = 138
 
 
 def log(*a):

@@ -174,6 +174,7 @@ long long estimate(const cgfdm_desc& d) {
     b += 3LL * (d.NX + d.NY + d.NZ + 18) * 4;
     b += (long long)d.NX * d.NY * 2 * 4;
     b += 256;  // uniform first metric pair
+    b += (long long)(d.NX + 6) * (d.NY + 6) * 3 * 4;  // per-column zeta
     return b;
 }
 
@@ -311,6 +312,7 @@ int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
         if ((rc = h->alloc(&h->W[b], (size_t)h->lay.size(9) * h->sb))) return rc;
     if ((rc = h->alloc(&h->cjm, (size_t)h->lay.size(13) * h->sb))) return rc;
     if ((rc = h->alloc(&h->m0, 16))) return rc;
+    if ((rc = h->alloc((void**)&h->zcol, sizeof(float) * 3 * (size_t)h->nx_ * h->ny_))) return rc;
     // PML slabs: x1 y1 z1 x2 y2 z2
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
@@ -383,7 +385,7 @@ int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& ho
     }
     SetupParams q = setup_params(h, dter);
     LCHK(launch_setup_medium(h->prec, q, dev, h->cjm, h->stream));
-    LCHK(launch_setup_metric(h->prec, q, h->cjm, h->stream));
+    LCHK(launch_setup_metric(h->prec, q, h->cjm, h->zcol, h->m0, h->stream));
     LCHK(launch_pair0_check(h->prec, q, h->cjm, h->m0, h->stream));
     if (h->d.free_surface) LCHK(launch_setup_rdz(q, dev, h->rdz, h->stream));
     if (h->d.check_cfl) {
@@ -415,6 +417,16 @@ int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& ho
     }
     if ((rc = h->settle())) return rc;
     cleanup();
+    if (!h->async) {
+        // the tile kernels take zeta from its per-column form (k_zcol), which every grid built
+        // here has; a grid without it would be an error of this library's set-up
+        int notLinear = 0;
+        HIPCHK(hipMemcpy(&notLinear, (const char*)h->m0 + 4, sizeof(int), hipMemcpyDeviceToHost));
+        if (notLinear) {
+            set_err("the metric zeta is not (k - 3) times a per-column value on this grid");
+            return CGFDM_EINVAL;
+        }
+    }
     h->modelSet = true;
     return CGFDM_OK;
 }

@@ -48,15 +48,19 @@ template <> struct SV<__bf16> {
 //
 // The metric/medium array (13 components) stores the reference components (0-8 metrics xi_x,
 // xi_y, xi_z, eta_x, eta_y, eta_z, zeta_x, zeta_y, zeta_z, 9 J, 10 mu, 11 lambda, 12 buoyancy)
-// in the slot order xi_x, eta_y, zeta_x, zeta_y, zeta_z, mu, lambda, buoyancy, xi_y, xi_z, eta_x,
-// eta_z, J. The grids of this library are terrain-following (x depends on i only, y on j only),
-// so x_eta = x_zeta = y_xi = y_zeta = 0 exactly (the averaged forward/backward differences of a
-// coordinate constant along an axis cancel bit for bit) and with them xi_y, xi_z, eta_x, eta_z,
-// whose adjugate terms are products of those zeros: the tile kernels read only the first four
-// pairs. atc() takes the reference numbering.
+// in the slot order xi_x, eta_y | mu, lambda | zeta_x, zeta_y | zeta_z, J | xi_y, xi_z |
+// eta_x, eta_z | buoyancy (the last, unpaired). The grids of this library are terrain-following
+// (x depends on i only, y on j only, z linear in k along every column), so x_eta = x_zeta = y_xi =
+// y_zeta = 0 exactly (the averaged forward/backward differences of a coordinate constant along an
+// axis cancel bit for bit) and with them xi_y, xi_z, eta_x, eta_z, whose adjugate terms are
+// products of those zeros; and zeta_x, zeta_y are (k - 3) times a per-column value, zeta_z is
+// constant along a column. The tile kernels therefore read per point only the first two pairs
+// and the buoyancy, and take zeta from three per-column values (StageParams::zcol). atc() takes
+// the reference numbering.
 constexpr int kBlk = 64;
 __host__ __device__ __forceinline__ constexpr int cjm_slot(int c) {
-    return c == 0 ? 0 : c == 1 ? 8 : c == 2 ? 9 : c == 3 ? 10 : c == 4 ? 1 : c == 5 ? 11 : c == 9 ? 12 : c <= 8 ? c - 4 : c - 5;
+    return c == 0 ? 0 : c == 1 ? 8 : c == 2 ? 9 : c == 3 ? 10 : c == 4 ? 1 : c == 5 ? 11 : c == 6 ? 4 : c == 7 ? 5
+         : c == 8 ? 6 : c == 9 ? 7 : c == 10 ? 2 : c == 11 ? 3 : 12;
 }
 struct Layout {
     int nx_, ny_, nz_;  // padded sizes (interior + 2*HALO)
@@ -167,6 +171,7 @@ struct StageParams {
     void* T;          // RK accumulator t
     const void* cjm;  // 13 planes: xi_x..zt_z, J, mu, lambda, buoyancy
     const void* m0;   // {int uniform, int, first metric pair}: the pair's value when uniform (or null)
+    const float* zcol;  // [3][ny_][nx_]: zeta_x / (k - 3), zeta_y / (k - 3), zeta_z of every column
     const float* pa[3];  // PML alpha per axis (padded length)
     const float* pb[3];  // 1/beta
     const float* pd[3];  // d/beta
@@ -225,7 +230,9 @@ enum Prec { P_F32 = 1, P_F16 = 2, P_BF16 = 3 };
 // ---- launchers (defined in the .hip files) ----
 int launch_setup_medium(int prec, const SetupParams& q, const MediumSrc& ms, void* cjm, hipStream_t s);
 int launch_vpmax(const SetupParams& q, const MediumSrc& ms, unsigned* out, hipStream_t s);
-int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s);
+// The metric array and the per-column zeta values of the tile kernels (zcol); m0 + 4 is raised
+// where a point's zeta does not have the per-column form.
+int launch_setup_metric(int prec, const SetupParams& q, void* cjm, float* zcol, void* m0, hipStream_t s);
 // Whether the first metric pair is the same at every interior point (m0: see setup.hip).
 int launch_pair0_check(int prec, const SetupParams& q, const void* cjm, void* m0, hipStream_t s);
 int launch_setup_rdz(const SetupParams& q, const MediumSrc& ms, float* rdz, hipStream_t s);

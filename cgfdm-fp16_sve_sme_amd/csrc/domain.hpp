@@ -58,6 +58,7 @@ struct cgfdm_domain {
     void* W[4] = {nullptr, nullptr, nullptr, nullptr};  // A, B, C, T
     void* cjm = nullptr;
     void* m0 = nullptr;     // 16 bytes: whether (xi_x, eta_y) is uniform, and its value
+    float* zcol = nullptr;  // [3][ny_][nx_] per-column zeta (k_zcol)
     bool pair0Stream = false;  // read the first metric pair per point even when uniform
     float* prof = nullptr;  // [ax bx dx][ay by dy][az bz dz]
     float* rdz = nullptr;
@@ -209,6 +210,7 @@ inline StageParams domain_stage_params(const cgfdm_domain* h) {
     p.T = h->W[3];
     p.cjm = h->cjm;
     p.m0 = h->pair0Stream ? nullptr : h->m0;
+    p.zcol = h->zcol;
     const int offs[3] = {0, 3 * h->nx_, 3 * (h->nx_ + h->ny_)};
     const int lens[3] = {h->nx_, h->ny_, h->nz_};
     for (int a = 0; a < 3; ++a) {

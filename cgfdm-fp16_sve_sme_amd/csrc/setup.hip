@@ -143,10 +143,37 @@ __global__ void k_vpmax(SetupParams q, MediumSrc ms, unsigned* out) {
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(vp));
 }
 
+// Per-column zeta: along a column of these grids z = -|D| + (h + |D|)/(NZ-1)*(k-3) is linear in
+// k and x, y do not depend on k, so z_zeta is constant, z_xi and z_eta are (k-3) times a
+// per-column value, and zeta_x = -z_xi/(x_xi z_zeta), zeta_y = -z_eta/(y_eta z_zeta) are
+// (k-3) times a per-column value while zeta_z = 1/z_zeta is constant. zcol = [3][ny_][nx_]
+// (FP32) holds those values, from the FP64 metric of the top interior layer, for the interior
+// columns (0 elsewhere); the tile kernels form zeta from them instead of reading it per point.
+__global__ void k_zcol(SetupParams q, float* __restrict__ zcol) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long pl = (long long)q.nx_ * q.ny_;
+    if (t >= pl) return;
+    const int i = (int)(t % q.nx_), j = (int)(t / q.nx_);
+    float a = 0.f, b = 0.f, g = 0.f;
+    if (i >= HALO && i < q.nx + HALO && j >= HALO && j < q.ny + HALO) {
+        const int kr = HALO + q.nz - 1;  // top interior layer, the largest k - 3
+        double m[9], J;
+        metric_at(q, i, j, kr, m, J);
+        a = (float)(m[6] / (double)(kr - HALO));
+        b = (float)(m[7] / (double)(kr - HALO));
+        g = (float)m[8];
+    }
+    zcol[t] = a;
+    zcol[pl + t] = b;
+    zcol[2 * pl + t] = g;
+}
+
 // Metric and J on the interior and, with a free surface, the mirrored values above it
-// (solve_coordinate_on_free_surface, contravariant.cpp:393-408: value at 2(_nz-1)-k).
+// (solve_coordinate_on_free_surface, contravariant.cpp:393-408: value at 2(_nz-1)-k). Each
+// interior point also checks that its zeta matches the per-column form (k_zcol) to 1e-5 of its
+// size and raises the int at m0 + 4 where it does not.
 template <class S>
-__global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
+__global__ void k_metric(SetupParams q, S* __restrict__ cjm, const float* __restrict__ zcol, void* m0) {
     const int _nz = q.nz_ - HALO;
     const int kTop = q.fs ? q.nz_ : _nz;
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -159,6 +186,14 @@ __global__ void k_metric(SetupParams q, S* __restrict__ cjm) {
     metric_at(q, i, j, kk, m, J);
     for (int c = 0; c < 9; ++c) cjm[q.lay.atc(c, i, j, k)] = SV<S>::st((float)m[c]);
     cjm[q.lay.atc(9, i, j, k)] = SV<S>::st((float)J);
+    if (k < _nz) {
+        const long long pl = (long long)q.nx_ * q.ny_, c0 = i + (long long)j * q.nx_;
+        const double kf = (double)(k - HALO), gz = zcol[2 * pl + c0];
+        const double ex = fabs(m[6] - zcol[c0] * kf), ey = fabs(m[7] - zcol[pl + c0] * kf), ez = fabs(m[8] - gz);
+        const double tol = 1e-5 * (fabs(m[6]) + fabs(m[7]) + fabs(m[8]));
+        if (!(ex <= tol && ey <= tol && ez <= tol))
+            __builtin_amdgcn_raw_buffer_store_b32(1u, make_rsrc((const int*)m0, 4), 4, 0, 0);
+    }
 }
 
 // Uniform first metric pair. The metric array's first variable pair is (xi_x, eta_y). On the
@@ -317,14 +352,15 @@ int launch_vpmax(const SetupParams& q, const MediumSrc& ms, unsigned* out, hipSt
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_setup_metric(int prec, const SetupParams& q, void* cjm, hipStream_t s) {
+int launch_setup_metric(int prec, const SetupParams& q, void* cjm, float* zcol, void* m0, hipStream_t s) {
     const int kTop = q.fs ? q.nz_ : q.nz_ - HALO;
     long long n = (long long)q.nx * q.ny * (kTop - HALO);
     const int bs = 256;
+    k_zcol<<<blocks_for((long long)q.nx_ * q.ny_, bs), bs, 0, s>>>(q, zcol);
     switch (prec) {
-        case P_F32: k_metric<float><<<blocks_for(n, bs), bs, 0, s>>>(q, (float*)cjm); break;
-        case P_F16: k_metric<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, (_Float16*)cjm); break;
-        case P_BF16: k_metric<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, (__bf16*)cjm); break;
+        case P_F32: k_metric<float><<<blocks_for(n, bs), bs, 0, s>>>(q, (float*)cjm, zcol, m0); break;
+        case P_F16: k_metric<_Float16><<<blocks_for(n, bs), bs, 0, s>>>(q, (_Float16*)cjm, zcol, m0); break;
+        case P_BF16: k_metric<__bf16><<<blocks_for(n, bs), bs, 0, s>>>(q, (__bf16*)cjm, zcol, m0); break;
         default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;

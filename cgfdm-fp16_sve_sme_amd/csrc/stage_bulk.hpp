@@ -220,6 +220,11 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const unsigned col = has ? (unsigned)L.col(9, i, j) * (unsigned)sizeof(S) : 0u;
     const unsigned col1 = has ? (unsigned)L.col1(9, i, j) * (unsigned)sizeof(S) : 0u;
     const unsigned colc = has ? (unsigned)L.col(13, i, j) * (unsigned)sizeof(S) : 0u;
+    const unsigned colc1 = has ? (unsigned)L.col1(13, i, j) * (unsigned)sizeof(S) : 0u;  // buoyancy
+    // zeta of the lane's column in its per-column form (k_zcol): zeta_x = zax (k - 3),
+    // zeta_y = zay (k - 3), zeta_z = zgz
+    const long long zpl = (long long)L.nx_ * L.ny_, zcx = has ? i + (long long)j * L.nx_ : 0;
+    const float zax = p.zcol[zcx], zay = p.zcol[zpl + zcx], zgz = p.zcol[2 * zpl + zcx];
 
     // z chunk, traversed in the direction FB3
     const int nk = ke - kb;
@@ -289,14 +294,17 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const float Crho = p.Crho;
     int k = k0;
 
-    // halo cells and metric/medium pairs of a plane: the metric array's first four pairs are
-    // (xi_x, eta_y), (zeta_x, zeta_y), (zeta_z, mu), (lambda, buoyancy); xi_y, xi_z, eta_x and
-    // eta_z are exactly zero on the terrain-following grids this library builds, so their terms
-    // are left out
+    // halo cells and metric/medium values of a plane: the metric array's first two pairs,
+    // (xi_x, eta_y) and (mu, lambda), and its unpaired last slot, the buoyancy; xi_y, xi_z, eta_x
+    // and eta_z are exactly zero on the terrain-following grids this library builds, so their
+    // terms are left out, and zeta comes from the per-column values above
+    using P1 = typename PR::one;
     W9<S> hv[HPT];
-    P2 M[4];
+    P2 M[2];
+    P1 Mb;
     [[maybe_unused]] W9<S> hvN[HPT];
-    [[maybe_unused]] P2 MN[4];
+    [[maybe_unused]] P2 MN[2];
+    [[maybe_unused]] P1 MbN;
     // The first pair (xi_x, eta_y) comes from the copy in p.m0 while the set-up found it uniform
     // over the grid: every lane and plane then loads the same cached line instead of streaming
     // the pair. The choice is of a descriptor and an offset, so no load sits in a branch; the
@@ -304,16 +312,16 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const int uni = p.m0 ? __builtin_amdgcn_readfirstlane(*(const int*)p.m0) : 0;
     const Rsrc rU = make_rsrc((const S*)(p.m0 ? (const char*)p.m0 + 8 : Cb), 2);
     const unsigned colc0 = uni ? 0u : colc;
-    auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[4]) {
+    auto fetch_plane = [&](int kk, W9<S> (&h)[HPT], P2 (&m)[2], P1& mb) {
         const Rsrc rW = make_rsrc((const S*)(Wb + (long long)kk * wpl), wrange);
 #pragma unroll
         for (int r = 0; r < HPT; ++r) h[r].template load<PB, CPH>(rW, hcol[r], hcol1[r]);
         const Rsrc rC = make_rsrc((const S*)(Cb + (long long)kk * cpl), crange);
         m[0] = PR::template ld<kCpDefault>(uni ? rU : rC, colc0, 0);
-#pragma unroll
-        for (int c = 1; c < 4; ++c) m[c] = PR::template ld<CPM>(rC, colc, c * PB);
+        m[1] = PR::template ld<CPM>(rC, colc, PB);
+        mb = PR::template ld1<CPM>(rC, colc1, 0);
     };
-    if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M);
+    if constexpr (PREFETCH) fetch_plane(min(max(k, 0), kmax), hv, M, Mb);
     // PML slabs of the box, per normal axis a: the x/y profile values of the lane, and the
     // auxiliary positions pos = base + kk*kstride with kk the plane index inside the slab's k range
     [[maybe_unused]] float pBeta[3] = {1.f, 1.f, 1.f}, pD[3] = {0.f, 0.f, 0.f}, pAD[3] = {0.f, 0.f, 0.f};
@@ -356,7 +364,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         auto* Lb = lds[it & 1];
         const int kc = min(max(k, 0), kmax);  // clamped plane (only the inactive iterations need it)
         const long long pw = (long long)kc * wpl;
-        if constexpr (!PREFETCH) fetch_plane(kc, hv, M);
+        if constexpr (!PREFETCH) fetch_plane(kc, hv, M, Mb);
         // stage the centre plane and its halo in LDS
         [[maybe_unused]] float cen[9];
         if constexpr (PACKED) {
@@ -383,7 +391,7 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         W9<S> nq;
         nq.template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)min(max(k + 4 * FB3, 0), kmax) * wpl), wrange), col,
                              col1);
-        if constexpr (PREFETCH) fetch_plane(min(max(k + FB3, 0), kmax), hvN, MN);
+        if constexpr (PREFETCH) fetch_plane(min(max(k + FB3, 0), kmax), hvN, MN, MbN);
         // RK registers of this plane: needed only by the final update
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         W9<S> ra, rt;
@@ -419,8 +427,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         }
         if (inside && active) {
             const float xix = PR::lo(M[0]), ety = PR::hi(M[0]);
-            const float ztx = PR::lo(M[1]), zty = PR::hi(M[1]), ztz = PR::lo(M[2]);
-            const float mu = PR::hi(M[2]) * DT, lam = PR::lo(M[3]) * DT, bu = PR::hi(M[3]) * (Crho * DT);
+            const float kz = (float)(kc - HALO);
+            const float ztx = zax * kz, zty = zay * kz, ztz = zgz;
+            const float mu = PR::lo(M[1]) * DT, lam = PR::hi(M[1]) * DT, bu = PR::f1(Mb) * (Crho * DT);
             // physical derivative d/dx_c from the three one-sided ones
             auto phys = [&](int c, float d0, float d1, float d2) {
                 return c == 0 ? xix * d0 + ztx * d2 : c == 1 ? ety * d1 + zty * d2 : ztz * d2;
@@ -574,7 +583,8 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
             for (int r = 0; r < HPT; ++r) hv[r] = hvN[r];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) M[c] = MN[c];
+            for (int c = 0; c < 2; ++c) M[c] = MN[c];
+            Mb = MbN;
         }
         k += FB3;
     };

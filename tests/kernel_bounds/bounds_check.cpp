@@ -243,6 +243,7 @@ int main(int argc, char** argv) {
     for (int b = 0; b < 4; ++b) h.W[b] = dev_alloc((size_t)wbytes, wn[b]);
     h.cjm = dev_alloc((size_t)h.lay.size(13) * h.sb, "cjm");
     h.m0 = dev_alloc(16, "m0");
+    h.zcol = (float*)dev_alloc(sizeof(float) * 3 * (size_t)h.nx_ * h.ny_, "zcol");
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(d, s);
     h.auxmem = (float*)dev_alloc((size_t)auxTotal * 9 * 3 * sizeof(float), "aux");
