@@ -169,7 +169,7 @@ long long estimate(const cgfdm_desc& d) {
     const Layout L = make_layout(d);
     int sb = d.precision == CGFDM_FP32 ? 4 : 2;
     long long b = (L.size(9) * 4 + L.size(13)) * sb;
-    for (int s = 0; s < 6; ++s) b += slab_points(d, s) * 9 * 3 * 4;
+    for (int s = 0; s < 6; ++s) b += slab_points(d, s) * 9 * 3 * sb;
     b += (long long)(d.NX + 6) * (d.NY + 6) * 18 * 4;
     b += 3LL * (d.NX + d.NY + d.NZ + 18) * 4;
     b += (long long)d.NX * d.NY * 2 * 4;
@@ -316,7 +316,7 @@ int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
     // PML slabs: x1 y1 z1 x2 y2 z2
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(h->d, s);
-    if (auxTotal > 0 && (rc = h->alloc((void**)&h->auxmem, (size_t)auxTotal * 9 * 3 * sizeof(float)))) return rc;
+    if (auxTotal > 0 && (rc = h->alloc((void**)&h->auxmem, (size_t)auxTotal * 9 * 3 * h->sb))) return rc;
     assign_slabs(h);
     if ((rc = h->alloc((void**)&h->prof, sizeof(float) * 3 * (h->nx_ + h->ny_ + h->nz_)))) return rc;
     if ((rc = h->alloc((void**)&h->rdz, sizeof(float) * 18 * (size_t)h->nx_ * h->ny_))) return rc;

@@ -136,13 +136,17 @@ template <> __device__ __forceinline__ void bst<__bf16>(Rsrc r, unsigned voff, i
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)v), r, (int)voff, soff, 0);
 }
 
-// One PML slab: auxiliary variables of the 9 fields, SoA [9][n], FP32, three RK registers
-// (current value, stage-0 copy, accumulator). Slab order x1,y1,z1,x2,y2,z2 as AUX6SURF
+// One PML slab: auxiliary variables of the 9 fields, SoA [9][n], three RK registers (current
+// value, stage-0 copy, accumulator) in the wavefield's storage type. A register holds Aux*DT,
+// the auxiliary's contribution to one stage's increment (h += -beta*Aux*DT, pml_deriv.cpp:
+// 209-237), so each auxiliary carries the scale of its own wavefield variable (Cv for the
+// velocities, Cs for the stresses) and stays inside that variable's 16-bit range; the reference
+// keeps Aux unscaled in FP32 (struct.h:271-277). Slab order x1,y1,z1,x2,y2,z2 as AUX6SURF
 // (reference struct.h:565-574).
 struct Slab {
-    float* a;
-    float* m;
-    float* t;
+    void* a;
+    void* m;
+    void* t;
     long long n;
     int on;
     int axis;   // 0 x, 1 y, 2 z

@@ -62,7 +62,7 @@ struct cgfdm_domain {
     bool pair0Stream = false;  // read the first metric pair per point even when uniform
     float* prof = nullptr;  // [ax bx dx][ay by dy][az bz dz]
     float* rdz = nullptr;
-    float* auxmem = nullptr;  // PML auxiliaries of every slab (FP32)
+    char* auxmem = nullptr;  // PML auxiliaries of every slab (storage type, Aux*DT)
     cg::Slab slab[6]{};
     float* pgv = nullptr;
     // stations (padded i, j, k triples)
@@ -175,8 +175,8 @@ int domain_set_model(cgfdm_domain* h, const double* terrain, const MediumSrc& ho
 int domain_bulk_tiling(const cgfdm_domain* h, int cfg, int& gx, int& gy, int& nch, int& kc);
 
 // Slab geometry and auxiliary registers of the six PML slabs (x1 y1 z1 x2 y2 z2), carved in that
-// order from h->auxmem (three FP32 registers of 9 variables each). h->d, h->nx..nz and h->auxmem
-// must be set; the slab sizes are slab_points.
+// order from h->auxmem (three registers of 9 variables each, h->sb bytes per value). h->d,
+// h->nx..nz, h->sb and h->auxmem must be set; the slab sizes are slab_points.
 inline void assign_slabs(cgfdm_domain* h) {
     long long off = 0;
     for (int s = 0; s < 6; ++s) {
@@ -190,9 +190,9 @@ inline void assign_slabs(cgfdm_domain* h) {
         X.n1 = X.axis == 1 ? h->d.nPML : h->ny;
         if (X.on) {
             X.a = h->auxmem + off;
-            X.m = X.a + 9 * X.n;
-            X.t = X.m + 9 * X.n;
-            off += 27 * X.n;
+            X.m = h->auxmem + off + 9 * X.n * h->sb;
+            X.t = h->auxmem + off + 18 * X.n * h->sb;
+            off += 27 * X.n * h->sb;
         }
     }
 }

@@ -327,9 +327,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     [[maybe_unused]] float pBeta[3] = {1.f, 1.f, 1.f}, pD[3] = {0.f, 0.f, 0.f}, pAD[3] = {0.f, 0.f, 0.f};
     [[maybe_unused]] long long pBase[3] = {0, 0, 0}, pKs[3] = {0, 0, 0}, xN[3] = {0, 0, 0};
     [[maybe_unused]] int pK0[3] = {0, 0, 0}, pKn[3] = {1, 1, 1}, xn4[3] = {0, 0, 0};
-    [[maybe_unused]] float* xA[3] = {nullptr, nullptr, nullptr};
-    [[maybe_unused]] float* xM[3] = {nullptr, nullptr, nullptr};
-    [[maybe_unused]] float* xT[3] = {nullptr, nullptr, nullptr};
+    [[maybe_unused]] S* xA[3] = {nullptr, nullptr, nullptr};
+    [[maybe_unused]] S* xM[3] = {nullptr, nullptr, nullptr};
+    [[maybe_unused]] S* xT[3] = {nullptr, nullptr, nullptr};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!((MASK >> a) & 1)) continue;
@@ -346,11 +346,11 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
             pD[a] = p.pd[a][c];
             pAD[a] = pD[a] + p.pa[a][c];
         }
-        xA[a] = X.a;
-        xM[a] = X.m;
-        xT[a] = X.t;
+        xA[a] = (S*)X.a;
+        xM[a] = (S*)X.m;
+        xT[a] = (S*)X.t;
         xN[a] = X.n;
-        xn4[a] = (int)(X.n * 4);  // bytes per auxiliary variable
+        xn4[a] = (int)(X.n * (long long)sizeof(S));  // bytes per auxiliary variable
     }
 
     auto step = [&](auto Rc, int it) {
@@ -397,27 +397,27 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         W9<S> ra, rt;
         if constexpr (IRK == 1 || IRK == 2) ra.template load<PB, CPS>(make_rsrc((const S*)(Ab + pw), wrange), col, col1);
         if constexpr (IRK >= 2) rt.template load<PB, CPS>(rT, col, col1);
-        // auxiliaries of this plane (lanes outside the box read element 0); the z profile
-        // values belong to the plane
+        // auxiliaries of this plane, Aux*DT in the storage type (lanes outside the box read
+        // element 0); the z profile values belong to the plane
         [[maybe_unused]] float xa[3][9], xm[3][9], xt[3][9];
         [[maybe_unused]] unsigned xoff[3] = {0u, 0u, 0u};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (!((MASK >> a) & 1)) continue;
             const int kk = min(max(kc - pK0[a], 0), pKn[a] - 1);
-            xoff[a] = inside ? (unsigned)((pBase[a] + (long long)kk * pKs[a]) * 4) : 0u;
-            const Rsrc rXa = make_rsrc(xA[a], 9 * xN[a]);
+            xoff[a] = inside ? (unsigned)((pBase[a] + (long long)kk * pKs[a]) * (long long)sizeof(S)) : 0u;
+            const Rsrc rXa = make_rsrc((const S*)xA[a], 9 * xN[a]);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) xa[a][v] = Pair<float>::template ld1<CPX>(rXa, xoff[a], v * xn4[a]);
+            for (int v = 0; v < 9; ++v) xa[a][v] = PR::f1(PR::template ld1<CPX>(rXa, xoff[a], v * xn4[a]));
             if constexpr (IRK == 1 || IRK == 2) {
-                const Rsrc rXm = make_rsrc(xM[a], 9 * xN[a]);
+                const Rsrc rXm = make_rsrc((const S*)xM[a], 9 * xN[a]);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xm[a][v] = Pair<float>::template ld1<CPX>(rXm, xoff[a], v * xn4[a]);
+                for (int v = 0; v < 9; ++v) xm[a][v] = PR::f1(PR::template ld1<CPX>(rXm, xoff[a], v * xn4[a]));
             }
             if constexpr (IRK >= 2) {
-                const Rsrc rXt = make_rsrc(xT[a], 9 * xN[a]);
+                const Rsrc rXt = make_rsrc((const S*)xT[a], 9 * xN[a]);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) xt[a][v] = Pair<float>::template ld1<CPX>(rXt, xoff[a], v * xn4[a]);
+                for (int v = 0; v < 9; ++v) xt[a][v] = PR::f1(PR::template ld1<CPX>(rXt, xoff[a], v * xn4[a]));
             }
             if (a == 2) {
                 pBeta[2] = p.pb[2][kc];
@@ -508,10 +508,10 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 if (!((MASK >> a) & 1)) continue;
-                // -beta*Aux*DT into the increment, then the single-direction right-hand side
-                // (with d, no beta) of the slab normal
+                // -beta*Aux*DT into the increment (the registers hold Aux*DT), then the
+                // single-direction right-hand side (with d, no beta) of the slab normal
 #pragma unroll
-                for (int v = 0; v < 9; ++v) h[v] += -pBeta[a] * xa[a][v] * DT;
+                for (int v = 0; v < 9; ++v) h[v] += -pBeta[a] * xa[a][v];
                 // metric row of the slab normal (zero entries as above)
                 const float mx = a == 0 ? xix : a == 2 ? ztx : 0.f;
                 const float my = a == 1 ? ety : a == 2 ? zty : 0.f;
@@ -530,27 +530,27 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
                 ha[6] = (my * b[0] + mx * b[1]) * mu;
                 ha[7] = (mz * b[0] + mx * b[2]) * mu;
                 ha[8] = (mz * b[1] + my * b[2]) * mu;
-                const Rsrc rXa = make_rsrc(xA[a], 9 * xN[a]);
-                const Rsrc rXm = make_rsrc(xM[a], 9 * xN[a]);
-                const Rsrc rXt = make_rsrc(xT[a], 9 * xN[a]);
+                const Rsrc rXa = make_rsrc((const S*)xA[a], 9 * xN[a]);
+                const Rsrc rXm = make_rsrc((const S*)xM[a], 9 * xN[a]);
+                const Rsrc rXt = make_rsrc((const S*)xT[a], 9 * xN[a]);
                 const unsigned xo = xoff[a];
                 const int x4 = xn4[a];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
-                    const float hav = ha[v] - pAD[a] * xa[a][v] * DT;
+                    // increment of Aux*DT: DT * (rhs*DT - (alpha+d)*Aux*DT)
+                    const float hav = (ha[v] - pAD[a] * xa[a][v]) * DT;
                     // as for the wavefield, the accumulator after stage 0 is rebuilt in stage 1
                     if constexpr (IRK == 0) {
-                        Pair<float>::template st1<CPX>(rXm, xo, v * x4, xa[a][v]);
-                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xa[a][v] + alpha2 * hav);
+                        PR::template st1<CPX>(rXm, xo, v * x4, xa[a][v]);
+                        PR::template st1<CPX>(rXa, xo, v * x4, xa[a][v] + alpha2 * hav);
                     } else if constexpr (IRK == 1) {
-                        Pair<float>::template st1<CPX>(rXt, xo, v * x4,
-                                                       (xm[a][v] + (xa[a][v] - xm[a][v]) * kT0) + beta2 * hav);
-                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + alpha3 * hav);
+                        PR::template st1<CPX>(rXt, xo, v * x4, (xm[a][v] + (xa[a][v] - xm[a][v]) * kT0) + beta2 * hav);
+                        PR::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + alpha3 * hav);
                     } else if constexpr (IRK == 2) {
-                        Pair<float>::template st1<CPX>(rXt, xo, v * x4, xt[a][v] + beta3 * hav);
-                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + hav);
+                        PR::template st1<CPX>(rXt, xo, v * x4, xt[a][v] + beta3 * hav);
+                        PR::template st1<CPX>(rXa, xo, v * x4, xm[a][v] + hav);
                     } else {
-                        Pair<float>::template st1<CPX>(rXa, xo, v * x4, xt[a][v] + beta4 * hav);
+                        PR::template st1<CPX>(rXa, xo, v * x4, xt[a][v] + beta4 * hav);
                     }
                 }
             }

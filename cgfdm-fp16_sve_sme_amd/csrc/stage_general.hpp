@@ -222,11 +222,15 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             const int k0 = (a == 2 ? c - X.lo : k - HALO);
             const long long pos = i0 + (long long)j0 * X.n0 + (long long)k0 * X.n0 * X.n1;
             const float beta = p.pb[a][c], dd = p.pd[a][c], alpha_d = dd + p.pa[a][c];
+            // the registers hold Aux*DT in the storage type
+            S* __restrict__ Xa = (S*)X.a;
+            S* __restrict__ Xm = (S*)X.m;
+            S* __restrict__ Xt = (S*)X.t;
             float aold[9];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) aold[v] = X.a[v * X.n + pos];
+            for (int v = 0; v < 9; ++v) aold[v] = SV<S>::ld(Xa[v * X.n + pos]);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) h[v] += -beta * aold[v] * DT;
+            for (int v = 0; v < 9; ++v) h[v] += -beta * aold[v];
             // single-direction right-hand side with d (pml_deriv.cpp:176-207)
             const float mx = M[3 * a], my = M[3 * a + 1], mz = M[3 * a + 2];
             float b[9];
@@ -243,9 +247,10 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
             r[6] = (my * b[0] + mx * b[1]) * mu;
             r[7] = (mz * b[0] + mx * b[2]) * mu;
             r[8] = (mz * b[1] + my * b[2]) * mu;
+            // right-hand side of Aux*DT, times DT below: DT * (rhs*DT - (alpha+d)*Aux*DT)
             float ha[9];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) ha[v] = (r[v] - alpha_d * aold[v]) * DT;
+            for (int v = 0; v < 9; ++v) ha[v] = r[v] * DT - alpha_d * aold[v];
             if (p.fs && a < 2 && k == _nz - 1) {
                 // pml_free_surface_x/y (pml_freeSurface.cpp:81-116)
                 const float* Mz = p.rdz + (a == 0 ? 0 : (long long)p.nx_ * p.ny_ * 9) + rdzi;
@@ -261,6 +266,8 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
                 ha[7] += ((zz * Vz0 + zx * Vz2) * mu) * DT;
                 ha[8] += ((zz * Vz1 + zy * Vz2) * mu) * DT;
             }
+#pragma unroll
+            for (int v = 0; v < 9; ++v) ha[v] *= DT;
             // wave_pml_rk0..3 (pml_rk.cpp:14-70), in place: each point owns its aux values; the
             // accumulator after stage 0 is rebuilt in stage 1 (kT0)
 #pragma unroll
@@ -268,17 +275,17 @@ __device__ void stage_point_general(const StageParams& p, int i, int j, int k) {
                 const long long q = v * X.n + pos;
                 if (p.irk == 0) {
                     const float m = aold[v];
-                    X.m[q] = m;
-                    X.a[q] = m + alpha2 * ha[v];
+                    Xm[q] = SV<S>::st(m);
+                    Xa[q] = SV<S>::st(m + alpha2 * ha[v]);
                 } else if (p.irk == 1) {
-                    const float m = X.m[q];
-                    X.t[q] = (m + (aold[v] - m) * kT0) + beta2 * ha[v];
-                    X.a[q] = m + alpha3 * ha[v];
+                    const float m = SV<S>::ld(Xm[q]);
+                    Xt[q] = SV<S>::st((m + (aold[v] - m) * kT0) + beta2 * ha[v]);
+                    Xa[q] = SV<S>::st(m + alpha3 * ha[v]);
                 } else if (p.irk == 2) {
-                    X.t[q] = X.t[q] + beta3 * ha[v];
-                    X.a[q] = X.m[q] + ha[v];
+                    Xt[q] = SV<S>::st(SV<S>::ld(Xt[q]) + beta3 * ha[v]);
+                    Xa[q] = SV<S>::st(SV<S>::ld(Xm[q]) + ha[v]);
                 } else {
-                    X.a[q] = X.t[q] + beta4 * ha[v];
+                    Xa[q] = SV<S>::st(SV<S>::ld(Xt[q]) + beta4 * ha[v]);
                 }
             }
         }

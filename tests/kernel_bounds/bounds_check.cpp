@@ -246,7 +246,7 @@ int main(int argc, char** argv) {
     h.zcol = (float*)dev_alloc(sizeof(float) * 3 * (size_t)h.nx_ * h.ny_, "zcol");
     long long auxTotal = 0;
     for (int s = 0; s < 6; ++s) auxTotal += slab_points(d, s);
-    h.auxmem = (float*)dev_alloc((size_t)auxTotal * 9 * 3 * sizeof(float), "aux");
+    h.auxmem = (char*)dev_alloc((size_t)auxTotal * 9 * 3 * h.sb, "aux");
     h.prof = (float*)dev_alloc(sizeof(float) * 3 * (h.nx_ + h.ny_ + h.nz_), "profiles");
     h.rdz = (float*)dev_alloc(sizeof(float) * 18 * (size_t)h.nx_ * h.ny_, "rdz");
     assign_slabs(&h);
