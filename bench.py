@@ -208,8 +208,8 @@ def serving_run(reqs, max_domains, steps_per_launch, paced=True, use_graph=True,
 
 
 def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batched", "one_at_a_time"),
-            least_work_first=False):
-    reqs = wl.serving_stream(n, seed)
+            least_work_first=False, rate_hz=20.0, sustained_hz=0.0):
+    reqs = wl.serving_stream(n, seed, rate_hz=rate_hz)
     lups = sum(r["grid"][0] * r["grid"][1] * r["grid"][2] * r["NT"] for _, r in reqs)
     out = {}
     for name, md in (("batched", max_domains), ("one_at_a_time", 1)):
@@ -229,9 +229,24 @@ def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batc
             f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s (queued p50 {np.percentile(qt, 50):.2f}s, "
             f"service p50 {np.percentile(sv, 50):.2f}s p99 {np.percentile(sv, 99):.2f}s), {lups / span / 1e9:.2f} GLUPS, "
             f"peak resident {st['peak_resident']}")
+    if sustained_hz > 0 and "batched" in modes:
+        # the same requests arriving at a rate the GPU sustains: latency is then mostly service
+        slow = wl.serving_stream(n, seed, rate_hz=sustained_hz)
+        lat, span, wall, st, qt, sv = serving_run(slow, max_domains, steps_per_launch, use_graph=use_graph,
+                                                  least_work_first=least_work_first)
+        out["batched_sustained"] = {"arrival_hz": sustained_hz, "solves_per_s": round(n / span, 4),
+                                    "p50_s": round(float(np.percentile(lat, 50)), 3),
+                                    "p99_s": round(float(np.percentile(lat, 99)), 3),
+                                    "makespan_s": round(span, 2), "peak_resident": st["peak_resident"],
+                                    "queue_p50_s": round(float(np.percentile(qt, 50)), 3),
+                                    "service_p50_s": round(float(np.percentile(sv, 50)), 3),
+                                    "service_p99_s": round(float(np.percentile(sv, 99)), 3)}
+        log(f"[serving batched, arrivals at {sustained_hz}/s] {n} requests in {span:.1f}s: p50 "
+            f"{np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s (service p50 {np.percentile(sv, 50):.2f}s), "
+            f"peak resident {st['peak_resident']}")
     out["requests"] = n
     out["admit_order"] = "least work first" if least_work_first else "arrival"
-    out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at 20/s, "
+    out["stream"] = (f"first {n} of the seeded 512-request stream (seed {seed}): Poisson arrivals at {rate_hz:g}/s, "
                      f"grids U[101,301]^2 x U[51,151], 2-5 random layers Vs 1500-4000 m/s, random double "
                      f"couples (M0 1e15 N m) 1-15 km deep, Ricker 0.5-3 Hz, 500-1500 steps, FP16 with scales chosen per request from the source, 5x5 surface "
                      f"stations")
@@ -330,6 +345,10 @@ def main():
     ap.add_argument("--serve-modes", default="batched,one_at_a_time", help="comma list of batched, one_at_a_time")
     ap.add_argument("--serve-order", default="arrival", choices=("arrival", "least-work"),
                     help="which waiting request a free slot takes")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the stream (1/s)")
+    ap.add_argument("--serve-sustained-rate", type=float, default=1.0,
+                    help="also serve the same requests batched at this arrival rate, one the GPU sustains "
+                         "(0: skip); reported as serving.batched_sustained")
     ap.add_argument("--finite-fault-steps", type=int, default=0,
                     help="also run the kinematic finite-fault workload (1201x1201x401, FP16) for this many "
                          "steps (4000: the whole rupture) and report it under 'finite_fault'")
@@ -357,8 +376,8 @@ def main():
         del model
     ffault = finite_fault_run(args.finite_fault_steps) if args.finite_fault_steps > 0 else None
     serve = serving(args.serve_requests, 7, args.serve_slots, args.serve_steps_per_launch, bool(args.serve_graph),
-                    tuple(args.serve_modes.split(",")), args.serve_order == "least-work") \
-        if args.serve_requests > 0 else None
+                    tuple(args.serve_modes.split(",")), args.serve_order == "least-work", args.serve_rate,
+                    args.serve_sustained_rate) if args.serve_requests > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(DH, DT, steps=args.cpu_steps)

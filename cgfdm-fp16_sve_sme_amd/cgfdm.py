@@ -16,7 +16,7 @@ FP32, FP16, BF16 = 1, 2, 3
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}
 
 STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECFL", -5: "EPML",
-          -6: "ESTATE", -7: "EIO"}
+          -6: "ESTATE", -7: "EIO", -8: "ENONFINITE"}
 
 
 class Desc(ctypes.Structure):
@@ -84,6 +84,8 @@ def lib():
         L.cgfdm_step.argtypes = [vp, i]
         L.cgfdm_sync.argtypes = [vp]
         L.cgfdm_current_step.argtypes = [vp]
+        L.cgfdm_set_check_interval.argtypes = [vp, i]
+        L.cgfdm_nonfinite_step.argtypes = [vp, P(i)]
         for n in ("cgfdm_get_stations", "cgfdm_get_pgv", "cgfdm_get_state", "cgfdm_set_state",
                   "cgfdm_get_cjm"):
             getattr(L, n).argtypes = [vp, vp]
@@ -185,6 +187,16 @@ class Domain:
     @property
     def it(self):
         return lib().cgfdm_current_step(self.h)
+
+    def set_check_interval(self, every):
+        """Scan the wavefield for non-finite values every `every` steps (0: never)."""
+        _chk(lib().cgfdm_set_check_interval(self.h, every))
+
+    def nonfinite_step(self):
+        """Step count after which the guard first found a non-finite value, or -1."""
+        out = ctypes.c_int(0)
+        _chk(lib().cgfdm_nonfinite_step(self.h, ctypes.byref(out)))
+        return out.value
 
     def stations(self):
         out = np.zeros((self.nst, self.desc.NT, 9), np.float32)

@@ -175,6 +175,7 @@ long long estimate(const cgfdm_desc& d) {
     b += (long long)d.NX * d.NY * 2 * 4;
     b += 256;  // uniform first metric pair
     b += (long long)(d.NX + 6) * (d.NY + 6) * 3 * 4;  // per-column zeta
+    b += 256;  // non-finite flag
     return b;
 }
 
@@ -321,6 +322,7 @@ int domain_init(cgfdm_domain* h, const cgfdm_desc* desc, int device) {
     if ((rc = h->alloc((void**)&h->prof, sizeof(float) * 3 * (h->nx_ + h->ny_ + h->nz_)))) return rc;
     if ((rc = h->alloc((void**)&h->rdz, sizeof(float) * 18 * (size_t)h->nx_ * h->ny_))) return rc;
     if ((rc = h->alloc((void**)&h->pgv, sizeof(float) * 2 * (size_t)h->nx * h->ny))) return rc;
+    if ((rc = h->alloc((void**)&h->nfFlag, sizeof(int)))) return rc;
     std::vector<float> prof;
     pml_profiles(h->d, prof);
     if ((rc = h->h2d(h->prof, prof.data(), prof.size() * sizeof(float)))) return rc;
@@ -690,6 +692,8 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
             }
             FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;
         }
+        if (h->checkEvery > 0 && (h->it + 1) % h->checkEvery == 0)
+            LCHK(launch_nonfinite(h->prec, h->W[0], h->lay, h->nfFlag, h->it + 1, h->stream));
         if (h->nst > 0 && h->it < h->d.NT)
             LCHK(launch_station(h->prec, h->W[0], h->lay, h->stIdx, h->nst, h->stWave, h->d.NT, h->it,
                                 (float)(1.0 / h->Cv), (float)(1.0 / h->Cs), h->stream));
@@ -701,10 +705,42 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
     return CGFDM_OK;
 }
 
+namespace {
+// After the domain's stream has drained: CGFDM_ENONFINITE once the guard has found a value that
+// is not finite.
+int nonfinite_status(cgfdm_domain* h) {
+    int f = 0;
+    HIPCHK(hipMemcpy(&f, h->nfFlag, sizeof(int), hipMemcpyDeviceToHost));
+    if (f != 0) {
+        set_err("non-finite wavefield value found after step %d (16-bit storage out of range? see cgfdm_source_scales)",
+                f);
+        return CGFDM_ENONFINITE;
+    }
+    return CGFDM_OK;
+}
+}  // namespace
+
 int cgfdm_sync(cgfdm_domain* h) {
     if (!h) { set_err("null handle"); return CGFDM_EINVAL; }
     HIPCHK(hipSetDevice(h->dev));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return nonfinite_status(h);
+}
+
+int cgfdm_set_check_interval(cgfdm_domain* h, int every) {
+    if (!h || every < 0) { set_err("bad arguments"); return CGFDM_EINVAL; }
+    if (h->pooled) { set_err("a served domain is checked by its server"); return CGFDM_ESTATE; }
+    h->checkEvery = every;
+    return CGFDM_OK;
+}
+
+int cgfdm_nonfinite_step(cgfdm_domain* h, int* step) {
+    if (!h || !step) { set_err("null argument"); return CGFDM_EINVAL; }
+    HIPCHK(hipSetDevice(h->dev));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int f = 0;
+    HIPCHK(hipMemcpy(&f, h->nfFlag, sizeof(int), hipMemcpyDeviceToHost));
+    *step = f != 0 ? f : -1;
     return CGFDM_OK;
 }
 
@@ -716,7 +752,7 @@ int cgfdm_get_stations(cgfdm_domain* h, float* out) {
     HIPCHK(hipSetDevice(h->dev));
     HIPCHK(hipMemcpyAsync(out, h->stWave, sizeof(float) * (size_t)h->nst * h->d.NT * 9, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    return CGFDM_OK;
+    return nonfinite_status(h);
 }
 
 int cgfdm_get_pgv(cgfdm_domain* h, float* out) {
@@ -724,7 +760,7 @@ int cgfdm_get_pgv(cgfdm_domain* h, float* out) {
     HIPCHK(hipSetDevice(h->dev));
     HIPCHK(hipMemcpyAsync(out, h->pgv, sizeof(float) * 2 * (size_t)h->nx * h->ny, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    return CGFDM_OK;
+    return nonfinite_status(h);
 }
 int cgfdm_get_slice(cgfdm_domain* h, int axis, int index, int var, float* out) {
     if (!h || !out || axis < 0 || axis > 2 || var < 0 || var > 8) { set_err("bad slice request"); return CGFDM_EINVAL; }

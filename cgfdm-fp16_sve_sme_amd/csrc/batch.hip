@@ -260,13 +260,13 @@ static int batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st, co
         sb = fk->side;
     }
     k_bulk_multi<S, TX, TY, IRK><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_pml_multi<S, IRK, 1><<<g.pmlBlocks[0], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 2><<<g.pmlBlocks[1], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 4><<<g.pmlBlocks[2], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 3><<<g.pmlBlocks[3], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 5><<<g.pmlBlocks[4], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 6><<<g.pmlBlocks[5], 256, 0, sb>>>(b);
-    k_pml_multi<S, IRK, 7><<<g.pmlBlocks[6], 256, 0, sb>>>(b);
+    k_pml_multi<S, IRK, 1><<<g.pmlBlocks[0], pml_tx(1) * pml_ty(1), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 2><<<g.pmlBlocks[1], pml_tx(2) * pml_ty(2), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 4><<<g.pmlBlocks[2], pml_tx(4) * pml_ty(4), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 3><<<g.pmlBlocks[3], pml_tx(3) * pml_ty(3), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 5><<<g.pmlBlocks[4], pml_tx(5) * pml_ty(5), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 6><<<g.pmlBlocks[5], pml_tx(6) * pml_ty(6), 0, sb>>>(b);
+    k_pml_multi<S, IRK, 7><<<g.pmlBlocks[6], pml_tx(7) * pml_ty(7), 0, sb>>>(b);
     k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, sb>>>(b);
     k_general_multi<S><<<g.genBlocks, kGenChunk, 0, sb>>>(b, IRK);
     if (sb != st) {
@@ -311,13 +311,13 @@ int batch_grid(int prec, int cfg, BatchGrid& g) {
     e1 = cfg ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 4, 1>, bt, 0)               \
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bulk_multi<S, 64, 8, 1>, bt, 0);              \
     e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ng, k_general_multi<S>, kGenChunk, 0);                  \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[0], k_pml_multi<S, 1, 1>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[1], k_pml_multi<S, 1, 2>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[2], k_pml_multi<S, 1, 4>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[3], k_pml_multi<S, 1, 3>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[4], k_pml_multi<S, 1, 5>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[5], k_pml_multi<S, 1, 6>, 256, 0); \
-    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[6], k_pml_multi<S, 1, 7>, 256, 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[0], k_pml_multi<S, 1, 1>, pml_tx(1) * pml_ty(1), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[1], k_pml_multi<S, 1, 2>, pml_tx(2) * pml_ty(2), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[2], k_pml_multi<S, 1, 4>, pml_tx(4) * pml_ty(4), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[3], k_pml_multi<S, 1, 3>, pml_tx(3) * pml_ty(3), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[4], k_pml_multi<S, 1, 5>, pml_tx(5) * pml_ty(5), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[5], k_pml_multi<S, 1, 6>, pml_tx(6) * pml_ty(6), 0); \
+    if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&npm[6], k_pml_multi<S, 1, 7>, pml_tx(7) * pml_ty(7), 0); \
     if (e2 == hipSuccess) e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nf, k_fs_multi<S, 1>, kFsTx * kFsTy, 0);
     switch (prec) {
         case P_F32: CG_OCC(float) break;

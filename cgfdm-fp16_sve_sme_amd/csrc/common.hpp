@@ -265,6 +265,8 @@ int launch_scale_mrate(int prec, const void* cjm, const Layout& L, const int* ij
 int launch_station(int prec, const void* A, const Layout& L, const int* ijk, int nst, float* wave, int NT, int it,
                    float c1, float c2, hipStream_t s);
 int launch_pgv(int prec, const void* A, const Layout& L, int nx, int ny, int k, float* pgv, float c, hipStream_t s);
+// Sets *flag to code (if still 0) when the 9-variable register A holds a value that is not finite.
+int launch_nonfinite(int prec, const void* A, const Layout& L, int* flag, int code, hipStream_t s);
 int launch_slice(int prec, const void* A, const Layout& L, int nx, int ny, int nz, int axis, int index, int var,
                  float c, float* out, hipStream_t s);
 // interior <-> float array of shape (V, NZ, NY, NX)

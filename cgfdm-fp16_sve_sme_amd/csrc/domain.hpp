@@ -65,6 +65,10 @@ struct cgfdm_domain {
     char* auxmem = nullptr;  // PML auxiliaries of every slab (storage type, Aux*DT)
     cg::Slab slab[6]{};
     float* pgv = nullptr;
+    // non-finite guard: every checkEvery steps (0: never) the register A is scanned; *nfFlag holds
+    // the step count after which the first non-finite value was found (0: none)
+    int* nfFlag = nullptr;
+    int checkEvery = 100;
     // stations (padded i, j, k triples)
     int* stIdx = nullptr;
     float* stWave = nullptr;

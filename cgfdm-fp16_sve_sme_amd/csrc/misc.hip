@@ -27,6 +27,24 @@ __global__ void k_multi_source(S* A, Layout L, const int* __restrict__ ijk, cons
     table_source_at<S>(A, L, ijk, mrate, npts, srcIt, tw, DT, p);
 }
 
+// Scan of a wavefield register for values that are not finite (infinite or NaN: the exponent
+// field all ones), as raw 32-bit words: one FP32 value or two 16-bit values per word, emask the
+// exponent field of every value in a word. A word with such a value writes `code` into *flag
+// unless the flag is already set (all lanes of one launch write the same code; launches on one
+// stream are ordered, so the first step found is kept). The padding and halo elements of the
+// register are zero or finite, so the whole array is scanned without its layout.
+__global__ void k_nonfinite(const unsigned* __restrict__ w, long long nwords, unsigned emask, int* flag, int code) {
+    const Rsrc rf = make_rsrc(flag, 1);
+    bool bad = false;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nwords;
+         q += (long long)gridDim.x * blockDim.x) {
+        const unsigned x = w[q];
+        const unsigned lo = emask & 0xffffu, hi = emask & 0xffff0000u;
+        bad |= (lo != 0u && (x & lo) == lo) || (hi != 0u && (x & hi) == hi);
+    }
+    if (bad && *(volatile const int*)flag == 0) __builtin_amdgcn_raw_buffer_store_b32((unsigned)code, rf, 0, 0, 0);
+}
+
 // Coefficients of the 7^3 point-source stencil from the stored Jacobian (no host round trip).
 template <class S>
 __global__ void k_source_coef(const S* __restrict__ cjm, Layout L, int SX, int SY, int SZ,
@@ -168,6 +186,16 @@ int launch_station(int prec, const void* A, const Layout& L, const int* ijk, int
                    float c1, float c2, hipStream_t s) {
     CG_DISPATCH(prec, k_station, nblk((long long)nst * 9, 256), 256, s, (const float*)A, (const _Float16*)A,
                 (const __bf16*)A, L, ijk, nst, wave, NT, it, c1, c2);
+    return CG_LAUNCH_OK;
+}
+
+int launch_nonfinite(int prec, const void* A, const Layout& L, int* flag, int code, hipStream_t s) {
+    const long long nwords = L.size(9) * storage_bytes(prec) / 4;
+    // FP32: one exponent field per word; FP16 (5 bits) and BF16 (8 bits): one per half word
+    const unsigned emask = prec == P_F32 ? 0x7f800000u : prec == P_F16 ? 0x7c007c00u : 0x7f807f80u;
+    const long long want = 2048LL * 256;  // grid-stride: at most 2048 workgroups
+    const unsigned grid = nblk(nwords < want ? nwords : want, 256);
+    k_nonfinite<<<grid, 256, 0, s>>>((const unsigned*)A, nwords, emask, flag, code);
     return CG_LAUNCH_OK;
 }
 

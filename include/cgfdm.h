@@ -32,7 +32,8 @@ enum cgfdm_status {
     CGFDM_ECFL = -4,     /* DT above the CFL limit (reference calc_CFL, 110)   */
     CGFDM_EPML = -5,     /* nPML larger than the grid (reference codes 110/130)*/
     CGFDM_ESTATE = -6,   /* call out of order (e.g. step before set_model)     */
-    CGFDM_EIO = -7       /* file could not be read or written                  */
+    CGFDM_EIO = -7,      /* file could not be read or written                  */
+    CGFDM_ENONFINITE = -8 /* the wavefield holds a value that is not finite    */
 };
 
 /* Storage precision of the wavefield, RK accumulator and packed metric/medium.
@@ -155,14 +156,26 @@ int cgfdm_set_stations(cgfdm_domain* h, const int* xyz, int n);
  * the body of the loop at propagate.cpp:142-198. Asynchronous on the domain's stream. */
 int cgfdm_step(cgfdm_domain* h, int nsteps);
 
-/* Waits for all work queued on the domain. */
+/* Waits for all work queued on the domain. Returns CGFDM_ENONFINITE once the non-finite guard
+ * (cgfdm_set_check_interval) has found a wavefield value that is not finite. */
 int cgfdm_sync(cgfdm_domain* h);
+
+/* Non-finite guard. No counterpart in the reference, which checks nothing (an FP16 overflow to
+ * Inf propagates silently through its time loop, propagate.cpp:142-198). Every `every` steps
+ * (default 100; 0 turns it off) cgfdm_step scans the wavefield after the step on the device
+ * (one read of the 9 stored planes, ~0.06 % of the step time at the default) and keeps the first
+ * step count after which a value was infinite or NaN; cgfdm_sync, cgfdm_get_stations and
+ * cgfdm_get_pgv then return CGFDM_ENONFINITE (the outputs are still copied).
+ * cgfdm_nonfinite_step waits for the domain and gives that step count, or -1. */
+int cgfdm_set_check_interval(cgfdm_domain* h, int every);
+int cgfdm_nonfinite_step(cgfdm_domain* h, int* step);
 
 /* Current time-step counter (number of completed steps). */
 int cgfdm_current_step(const cgfdm_domain* h);
 
 /* Station seismograms, n x NT x 9 float (Vx,Vy,Vz,Txx,Tyy,Tzz,Txy,Txz,Tyz), physical
- * units; the layout of station_mpi_X_Y_Z.bin after its int32 header (station.cpp:382-405). */
+ * units; the layout of station_mpi_X_Y_Z.bin after its int32 header (station.cpp:382-405).
+ * This and cgfdm_get_pgv return CGFDM_ENONFINITE as cgfdm_sync does. */
 int cgfdm_get_stations(cgfdm_domain* h, float* out);
 
 /* Peak ground velocity on the free surface, NY x NX x 2 float (PGVh, PGV);

@@ -14,6 +14,13 @@
 //                     [--steps N] [--shrink-arrays]
 // --shrink-arrays registers the wavefield arrays one k-plane short of their layout (a planted
 // mistake: the run must then report accesses outside the arrays).
+// --coverage also checks what the stage kernels write: the stage output register is filled with
+// a sentinel byte pattern before every stage, and afterwards every interior element of its 9
+// variables must have been written and no other element (halo, row padding) touched; likewise the
+// PML auxiliaries' stage-0 copy in stage 0 and their accumulator in stage 1, at every slab point.
+// Stores move their bytes in this mode; loads through descriptors still return 0.
+// --plant-coverage (with --coverage) resets one interior output element to the sentinel after
+// each stage (a planted mistake: the check must then report an element that was not written).
 #include <signal.h>
 #include <sys/mman.h>
 #include <unistd.h>
@@ -32,6 +39,7 @@
 thread_local emu_dim threadIdx, blockIdx, blockDim, gridDim;
 thread_local long long t_emu_accesses = 0;
 emu_log g_emu;
+bool g_emu_store_data = false;
 
 using namespace cg;
 
@@ -164,6 +172,75 @@ void stage(const StageParams& p, bool boundary_only) {
 const int FB_TABLE[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
                             {-1, 1, -1},  {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};
 
+// Write coverage (--coverage): sentinel-filled output, checked after each stage.
+bool g_cover = false, g_plant_cover = false;
+long long g_cover_bad = 0;
+
+void fill_sentinel(void* p, size_t n) { std::memset(p, 0xff, n); }
+
+// Elements of an array of `n` elements of `sb` bytes that still hold the sentinel.
+inline bool is_sentinel(const char* e, int sb) {
+    for (int b = 0; b < sb; ++b)
+        if ((unsigned char)e[b] != 0xff) return false;
+    return true;
+}
+
+// The stage output register W (9 variables): every interior element written, nothing else.
+// Afterwards the whole array is zeroed, so no sentinel reaches a later stage as input.
+void check_wout(const cgfdm_domain& h, void* W, const char* what) {
+    const Layout& L = h.lay;
+    const int sb = h.sb;
+    char* b = (char*)W;
+    if (g_plant_cover) {
+        const long long q = L.at(9, 4, HALO + h.nx / 2, HALO + h.ny / 2, HALO + h.nz / 2);
+        std::memset(b + q * sb, 0xff, sb);
+    }
+    long long miss = 0, written = 0;
+    for (int k = HALO; k < HALO + h.nz; ++k)
+        for (int j = HALO; j < HALO + h.ny; ++j)
+            for (int i = HALO; i < HALO + h.nx; ++i)
+                for (int v = 0; v < 9; ++v)
+                    if (is_sentinel(b + L.at(9, v, i, j, k) * sb, sb)) {
+                        if (miss++ < 4)
+                            fprintf(stderr, "coverage: %s: interior element (v %d, i %d, j %d, k %d) not written\n", what, v,
+                                    i, j, k);
+                    }
+    const long long n = L.size(9);
+    for (long long e = 0; e < n; ++e)
+        if (!is_sentinel(b + e * sb, sb)) ++written;
+    const long long interior = 9LL * h.nx * h.ny * h.nz;
+    if (written != interior - miss) {
+        fprintf(stderr, "coverage: %s: %lld elements written, %lld interior elements\n", what, written, interior - miss);
+        ++g_cover_bad;
+    }
+    if (miss) ++g_cover_bad;
+    std::memset(b, 0, (size_t)n * sb);
+}
+
+// One PML register (9 variables at every point of every slab that is on) fully written.
+void check_aux(const cgfdm_domain& h, int reg, const char* what) {
+    for (int s = 0; s < 6; ++s) {
+        const Slab& X = h.slab[s];
+        if (!X.on) continue;
+        char* b = (char*)(reg == 1 ? X.m : X.t);
+        long long miss = 0;
+        for (long long e = 0; e < 9 * X.n; ++e)
+            if (is_sentinel(b + e * h.sb, h.sb)) ++miss;
+        if (miss) {
+            fprintf(stderr, "coverage: %s of slab %d: %lld of %lld elements not written\n", what, s, miss, 9 * X.n);
+            ++g_cover_bad;
+        }
+        std::memset(b, 0, (size_t)(9 * X.n) * h.sb);
+    }
+}
+
+void fill_aux(const cgfdm_domain& h, int reg) {
+    for (int s = 0; s < 6; ++s) {
+        const Slab& X = h.slab[s];
+        if (X.on) fill_sentinel(reg == 1 ? X.m : X.t, (size_t)(9 * X.n) * h.sb);
+    }
+}
+
 template <class S>
 int run(cgfdm_domain& h, int cfg, int kc, bool boundary_only, int steps) {
     StageParams p = domain_stage_params(&h);
@@ -180,11 +257,23 @@ int run(cgfdm_domain& h, int cfg, int kc, bool boundary_only, int steps) {
                     p.Wout = stage_out(&h, irk);
                     p.irk = irk;
                     p.FB1 = FB1; p.FB2 = FB2; p.FB3 = FB3;
+                    const size_t wbytes = (size_t)h.lay.size(9) * h.sb;
+                    if (g_cover) {
+                        fill_sentinel(p.Wout, wbytes);
+                        if (irk == 0) fill_aux(h, 1);
+                        if (irk == 1) fill_aux(h, 2);
+                    }
                     switch (irk) {
                         case 0: stage<S, 0>(p, boundary_only); break;
                         case 1: stage<S, 1>(p, boundary_only); break;
                         case 2: stage<S, 2>(p, boundary_only); break;
                         default: stage<S, 3>(p, boundary_only); break;
+                    }
+                    if (g_cover) {
+                        static const char* wn[4] = {"stage 0 output", "stage 1 output", "stage 2 output", "stage 3 output"};
+                        check_wout(h, p.Wout, wn[irk]);
+                        if (irk == 0) check_aux(h, 1, "stage 0: PML stage-0 copy");
+                        if (irk == 1) check_aux(h, 2, "stage 1: PML accumulator");
                     }
                     FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;
                 }
@@ -227,6 +316,8 @@ int main(int argc, char** argv) {
     for (int a = 9; a < argc; ++a) {
         if (!strcmp(argv[a], "--boundary-only")) boundary_only = true;
         if (!strcmp(argv[a], "--shrink-arrays")) shrink = true;
+        if (!strcmp(argv[a], "--coverage")) g_cover = g_emu_store_data = true;
+        if (!strcmp(argv[a], "--plant-coverage")) g_plant_cover = true;
         if (!strcmp(argv[a], "--steps") && a + 1 < argc) steps = atoi(argv[++a]);
     }
 
@@ -259,5 +350,6 @@ int main(int argc, char** argv) {
     else rc = run<__bf16>(h, cfg, kc, boundary_only, steps);
     const long long oob = g_emu.oob.load(), wild = g_emu.wild.load();
     printf("accesses %lld, past a descriptor %lld, outside every array %lld\n", g_emu.accesses.load(), oob, wild);
-    return rc || oob || wild ? 1 : 0;
+    if (g_cover) printf("coverage: %lld failed checks\n", g_cover_bad);
+    return rc || oob || wild || g_cover_bad ? 1 : 0;
 }

@@ -82,7 +82,17 @@ inline void emu_access(const emu_rsrc& r, int voff, int soff, int size) {
 #define __builtin_amdgcn_raw_buffer_load_b32(r, v, s, cp) (emu_access((r), (v), (s), 4), 0u)
 #define __builtin_amdgcn_raw_buffer_load_b64(r, v, s, cp) \
     (emu_access((r), (v), (s), 8), (unsigned __attribute__((ext_vector_type(2)))){0u, 0u})
-#define __builtin_amdgcn_raw_buffer_store_b16(x, r, v, s, cp) ((void)(x), emu_access((r), (v), (s), 2))
-#define __builtin_amdgcn_raw_buffer_store_b32(x, r, v, s, cp) ((void)(x), emu_access((r), (v), (s), 4))
-#define __builtin_amdgcn_raw_buffer_store_b64(x, r, v, s, cp) ((void)(x), emu_access((r), (v), (s), 8))
+// Stores: with g_emu_store_data set (the write-coverage mode) an in-range store also writes its
+// bytes, so the caller can see which elements a stage wrote.
+extern bool g_emu_store_data;
+template <class T> inline void emu_store(const emu_rsrc& r, int voff, int soff, T x) {
+    emu_access(r, voff, soff, (int)sizeof(T));
+    const long long off = (long long)(unsigned)voff + (long long)soff;
+    if (g_emu_store_data && soff >= 0 && off + (long long)sizeof(T) <= r.bytes &&
+        emu_registered(r.base + off, (long long)sizeof(T)))
+        std::memcpy((void*)(r.base + off), &x, sizeof(T));
+}
+#define __builtin_amdgcn_raw_buffer_store_b16(x, r, v, s, cp) emu_store((r), (v), (s), (unsigned short)(x))
+#define __builtin_amdgcn_raw_buffer_store_b32(x, r, v, s, cp) emu_store((r), (v), (s), (unsigned)(x))
+#define __builtin_amdgcn_raw_buffer_store_b64(x, r, v, s, cp) emu_store((r), (v), (s), (x))
 #define __builtin_amdgcn_readfirstlane(x) (x)

@@ -95,7 +95,10 @@ def test_no_fallback_when_library_missing(gpu, monkeypatch):
 
 @pytest.mark.parametrize("fs,pml,npml,shape", [(1, 1, 12, (60, 50, 40)), (0, 1, 6, (30, 26, 20)),
                                                (1, 0, 6, (20, 18, 12)), (0, 0, 5, (16, 16, 16)),
-                                               (1, 1, 8, (18, 40, 30)), (1, 1, 4, (9, 9, 10))])
+                                               (1, 1, 8, (18, 40, 30)), (1, 1, 4, (9, 9, 10)),
+                                               (1, 1, 12, (150, 40, 30)), (1, 1, 12, (101, 30, 20)),
+                                               (0, 1, 6, (270, 30, 24)), (1, 1, 12, (1001, 30, 30)),
+                                               (1, 1, 12, (200, 30, 30))])
 def test_stage_decomposition_covers_interior_once(gpu, fs, pml, npml, shape):
     """Bulk box, PML face/edge/corner boxes, the free-surface box and general boxes tile the
     interior exactly once; a face box lies inside exactly one slab (its normal axis), an edge or

@@ -35,14 +35,27 @@ def checker(tmp_path_factory):
     ((150, 37, 29), 6, "fp32", 0, 1),    # no free surface: top PML slab, general boxes absent
     ((41, 130, 33), 12, "bf16", 1, 1),   # bulk box narrower than one tile in x
     ((270, 30, 24), 6, "fp16", 1, 8),    # three 128-wide bulk tiles; every MacCormack pattern
+    ((40, 36, 30), 8, "fp32", 1, 1),     # the command-line driver's test grid
+    ((200, 40, 36), 12, "fp16", 1, 1),   # right x slab across a layout-block boundary
 ])
 def test_stage_kernels_stay_in_bounds(checker, grid, npml, prec, fs, steps):
+    """Every access inside its array, and (--coverage) every interior point of the stage output
+    and every point of the PML registers written, nothing else."""
     env = dict(os.environ, OMP_NUM_THREADS=str(min(8, os.cpu_count() or 1)))
-    r = subprocess.run([checker, *map(str, grid), str(npml), prec, str(fs), "-1", "0", "--steps", str(steps)],
-                       capture_output=True, text=True, env=env, timeout=600)
+    r = subprocess.run([checker, *map(str, grid), str(npml), prec, str(fs), "-1", "0", "--steps", str(steps),
+                        "--coverage"], capture_output=True, text=True, env=env, timeout=600)
     print(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr[-2000:]
     assert "past a descriptor 0, outside every array 0" in r.stdout
+    assert "coverage: 0 failed checks" in r.stdout
+
+
+def test_coverage_check_catches_a_planted_gap(checker):
+    """An interior output element reset to the sentinel after each stage must be reported."""
+    r = subprocess.run([checker, "80", "40", "20", "8", "fp16", "1", "3", "0", "--coverage", "--plant-coverage"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode != 0
+    assert "not written" in r.stderr
 
 
 def test_checker_catches_a_planted_overrun(checker, tmp_path):
