@@ -17,6 +17,7 @@ PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}
 
 STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECFL", -5: "EPML",
           -6: "ESTATE", -7: "EIO", -8: "ENONFINITE"}
+STATUS_CODES = {v: k for k, v in STATUS.items()}
 
 
 class Desc(ctypes.Structure):
@@ -101,6 +102,7 @@ def lib():
         L.cgfdm_finite_fault_map.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, ll,
                                              P(ll), P(i), P(f)]
         L.cgfdm_aeqd_forward.argtypes = [ctypes.c_double, ctypes.c_double, ll, vp, vp, vp, vp]
+        L.cgfdm_point_source_scales.argtypes = [ctypes.c_double] * 4 + [P(ctypes.c_double)] * 2
         L.cgfdm_source_scales.argtypes = [ctypes.c_double, vp, vp, ll, i, f, ctypes.c_double, P(ctypes.c_double),
                                           P(ctypes.c_double)]
         L.cgfdm_server_create.argtypes = [vp, P(vp)]
@@ -324,6 +326,16 @@ def source_scales(DH, xyz=None, table=None, dt=None, mu_max=None):
             raise ValueError("xyz and table disagree on the number of nodes")
         _chk(lib().cgfdm_source_scales(float(DH), _p(x), _p(t), t.shape[1], t.shape[0], float(dt), float(mu_max),
                                        ctypes.byref(cv), ctypes.byref(cs)))
+    return cv.value, cs.value
+
+
+def point_source_scales(DH, fc, mu_src, mu_max):
+    """(Cv, Cs) of 16-bit storage for the explosive point source of peak frequency fc in a
+    medium whose shear modulus is mu_src at the source and mu_max at most
+    (cgfdm_point_source_scales)."""
+    cv, cs = ctypes.c_double(), ctypes.c_double()
+    _chk(lib().cgfdm_point_source_scales(float(DH), float(fc), float(mu_src), float(mu_max), ctypes.byref(cv),
+                                         ctypes.byref(cs)))
     return cv.value, cs.value
 
 

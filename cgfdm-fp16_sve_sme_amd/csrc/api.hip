@@ -460,6 +460,19 @@ int cgfdm_source_scales(double DH, const int* xyz, const float* rate, long long 
     return CGFDM_OK;
 }
 
+int cgfdm_point_source_scales(double DH, double fc, double mu_src, double mu_max, double* Cv, double* Cs) {
+    if (!Cv || !Cs || !(DH > 0) || !(fc > 0) || !(mu_src > 0) || !(mu_max > 0)) {
+        set_err("point_source_scales needs DH, fc, mu_src and mu_max > 0");
+        return CGFDM_EINVAL;
+    }
+    point_source_scales(DH, fc, mu_src, mu_max, *Cv, *Cs);
+    if (!(*Cv > 0) || !std::isfinite(*Cv)) {
+        set_err("point_source_scales: no finite scale");
+        return CGFDM_EINVAL;
+    }
+    return CGFDM_OK;
+}
+
 int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) {
     if (!out) { set_err("null output handle"); return CGFDM_EINVAL; }
     *out = nullptr;

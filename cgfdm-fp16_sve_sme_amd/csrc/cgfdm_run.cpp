@@ -483,8 +483,16 @@ int main(int argc, char** argv) {
         double mu = 0.0;
         for (size_t q = 0; q < nin; ++q) mu = std::max(mu, (double)rho[q] * vs[q] * vs[q]);
         const bool table = P.useMultiSource && nsrc > 0;
-        if ((rc = cgfdm_source_scales(d.DH, table ? sxyz.data() : nullptr, table ? srate.data() : nullptr,
-                                      table ? nsrc : 0, table ? snt : 0, table ? sdt : 0.f, mu, &d.Cv, &d.Cs))) {
+        if (!table && P.useSingleSource) {
+            // the medium at the source cell
+            const size_t q = (size_t)P.sourceX + (size_t)P.sourceY * NX + (size_t)P.sourceZ * NX * NY;
+            const double muSrc = q < nin ? (double)rho[q] * vs[q] * vs[q] : mu;
+            rc = cgfdm_point_source_scales(d.DH, P.rickerfc, muSrc, mu, &d.Cv, &d.Cs);
+        } else {
+            rc = cgfdm_source_scales(d.DH, table ? sxyz.data() : nullptr, table ? srate.data() : nullptr,
+                                     table ? nsrc : 0, table ? snt : 0, table ? sdt : 0.f, mu, &d.Cv, &d.Cs);
+        }
+        if (rc) {
             std::fprintf(stderr, "scales: %s\n", cgfdm_last_error());
             return 1;
         }

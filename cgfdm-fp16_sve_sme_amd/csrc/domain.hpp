@@ -115,9 +115,9 @@ constexpr double kRefCv = 5.0e3, kRefCs = 1.0e-6, kRefCrho = 1.0e6;
 // and 1e4 leaves a further 6x below the FP16 maximum (65504).
 constexpr double kVelTarget = 1.0e4, kStressTarget = 1.0e4;
 
-// Scale factors (Cv, Cs) of 16-bit storage chosen from the source. Point source (rate == null):
-// the reference's (5e3, 1e-6), calibrated for its fixed 1e16 N m Ricker source in 100 m cells,
-// times (DH/100)^2 (the near-field velocity of a fixed moment rate falls as 1/DH^2). Moment-rate
+// Scale factors (Cv, Cs) of 16-bit storage chosen from the source. Without a table (rate ==
+// null): the reference's (5e3, 1e-6) times (DH/100)^2 (point_source_scales below estimates the
+// point source from its wavelet and the medium instead). Moment-rate
 // table (nt x npts x 6, tensor * area * slip rate, as cgfdm_set_source_table takes it, at grid
 // nodes xyz): the peak particle velocity is estimated as the largest, over nodes p and radii
 // R = DH, 2 DH, ..., 16 DH, of 2 * sum_{|q - p| <= R} max_t|rate_q| / (4 pi R^2). At R = DH this
@@ -171,6 +171,39 @@ inline void source_scales(double DH, const int* xyz, const float* rate, long lon
         if (T > 0.0) Cv = std::min(Cv, ratio * kStressTarget / T);
     }
     Cs = Cv / ratio;
+}
+
+// Scale factors of 16-bit storage for the explosive point source (singleSource.cpp:29-127: the
+// Ricker moment rate s(t) = M0 2 pi fc r (2 r^2 - 3) exp(-r^2), r = pi fc (t - 1.2/fc), M0 =
+// 1e16 N m, spread over the 7^3 Gaussian weights GAUSS_FUN(d, 1.5, 0) / 0.998125703461425 into
+// Txx, Tyy and Tzz): the rule of source_scales applied to the equivalent table, one node per
+// stencil point holding rate = s(t) * weight / mu_src on the three normal components, sampled at
+// fc/40 over the wavelet. mu_src is the shear modulus at the source (the smallest of the medium
+// where that is not known: a softer medium moves faster for the same moment rate), mu_max the
+// largest of the medium.
+inline void point_source_scales(double DH, double fc, double mu_src, double mu_max, double& Cv, double& Cs) {
+    const double pi = 3.14159265358979323846, M0 = 1.0e16;
+    const double dt = 1.0 / (40.0 * fc);
+    const int nt = (int)std::ceil(2.4 / fc / dt) + 1;
+    double w[7];
+    for (int a = 0; a < 7; ++a) w[a] = std::exp(-0.5 * (a - 3) * (a - 3) / (1.5 * 1.5));
+    double wsum = 0.0;
+    for (int a = 0; a < 7; ++a)
+        for (int b = 0; b < 7; ++b)
+            for (int c = 0; c < 7; ++c) wsum += w[a] * w[b] * w[c];
+    std::vector<int> xyz(343 * 3);
+    std::vector<float> rate((size_t)nt * 343 * 6, 0.0f);
+    for (int q = 0; q < 343; ++q) {
+        const int a = q % 7, b = (q / 7) % 7, c = q / 49;
+        xyz[3 * q] = a - 3; xyz[3 * q + 1] = b - 3; xyz[3 * q + 2] = c - 3;
+        const double wq = w[a] * w[b] * w[c] / wsum;
+        for (int t = 0; t < nt; ++t) {
+            const double r = pi * fc * (t * dt - 1.2 / fc);
+            const double sv = M0 * 2.0 * pi * fc * r * (2.0 * r * r - 3.0) * std::exp(-r * r);
+            for (int comp = 0; comp < 3; ++comp) rate[((size_t)t * 343 + q) * 6 + comp] = (float)(sv * wq / mu_src);
+        }
+    }
+    source_scales(DH, xyz.data(), rate.data(), 343, nt, dt, mu_max, Cv, Cs);
 }
 
 // Set-up entry points shared by the C ABI and the server.

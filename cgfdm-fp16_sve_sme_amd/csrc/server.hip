@@ -20,6 +20,7 @@
 // domain's last use.
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -458,6 +459,14 @@ void cgfdm_server::loop() {
             const size_t nst = r.st.size() / 3;
             r.stations.assign(R.resSt, R.resSt + nst * 9 * (size_t)r.desc.NT);
             r.pgv.assign(R.resPgv, R.resPgv + 2 * (size_t)r.desc.NX * r.desc.NY);
+            // non-finite guard of served requests: judged on the copied results
+            bool finite = true;
+            for (float x : r.stations) finite = finite && std::isfinite(x);
+            for (float x : r.pgv) finite = finite && std::isfinite(x);
+            if (!finite) {
+                r.status = CGFDM_ENONFINITE;
+                r.err = "non-finite seismogram or PGV value (16-bit storage out of range?)";
+            }
             r.tDone = Clock::now();
             devArena.release(R.devOff, r.devBytes);
             hostArena.release(R.hostOff, r.hostBytes);
@@ -607,21 +616,29 @@ int cgfdm_server_submit(cgfdm_server* s, const cgfdm_request* q, long long* id) 
         return CGFDM_EINVAL;
     }
     if (d.precision != CGFDM_FP32 && !(d.Cv > 0) && !(d.Cs > 0)) {
-        // per-request scales from the source (cgfdm_source_scales), with the largest shear
-        // modulus of the request's medium
-        double mu = 0.0;
+        // per-request scales from the source (cgfdm_source_scales; point source:
+        // cgfdm_point_source_scales with the smallest shear modulus of the medium), with the
+        // largest shear modulus of the request's medium
+        double mu = 0.0, muMin = 1e300;
         if (r->nl > 0) {
-            for (int l = 0; l < r->nl; ++l)
-                mu = std::max(mu, (double)r->layers[3 * r->nl + l] * r->layers[r->nl + l] * r->layers[r->nl + l]);
+            for (int l = 0; l < r->nl; ++l) {
+                const double m = (double)r->layers[3 * r->nl + l] * r->layers[r->nl + l] * r->layers[r->nl + l];
+                mu = std::max(mu, m);
+                muMin = std::min(muMin, m);
+            }
             mu = std::max(mu, 2670.0 * 3464.0 * 3464.0);  // points outside every layer keep the default medium
         } else {
-            for (size_t t = 0; t < nin; ++t) mu = std::max(mu, (double)r->rho[t] * r->vs[t] * r->vs[t]);
+            for (size_t t = 0; t < nin; ++t) {
+                const double m = (double)r->rho[t] * r->vs[t] * r->vs[t];
+                mu = std::max(mu, m);
+                muMin = std::min(muMin, m);
+            }
         }
         double Cv = 0.0, Cs = 0.0;
         if (q->source == 2)
             source_scales(d.DH, r->tabXyz.data(), r->tabRate.data(), r->tabNpts, r->tabNt, r->tabDt, mu, Cv, Cs);
-        else
-            source_scales(d.DH, nullptr, nullptr, 0, 0, 0.0, mu, Cv, Cs);
+        else if (muMin > 0 && muMin < 1e300)
+            point_source_scales(d.DH, r->fc, muMin, mu, Cv, Cs);
         if (Cv > 0 && std::isfinite(Cv)) {
             r->desc.Cv = Cv;
             r->desc.Cs = Cs;
@@ -683,7 +700,7 @@ int cgfdm_server_result(cgfdm_server* s, long long id, float* stations, float* p
         info->device_bytes = (long long)r->devBytes;
         std::snprintf(info->error, sizeof(info->error), "%s", r->err.c_str());
     }
-    if (r->status == CGFDM_OK) {
+    if (r->status == CGFDM_OK || r->status == CGFDM_ENONFINITE) {
         if (stations && !r->stations.empty()) std::memcpy(stations, r->stations.data(), r->stations.size() * 4);
         if (pgv) std::memcpy(pgv, r->pgv.data(), r->pgv.size() * 4);
     }

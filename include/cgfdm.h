@@ -103,6 +103,13 @@ long long cgfdm_estimate_bytes(const cgfdm_desc* desc);
 int cgfdm_source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, float dt,
                         double mu_max, double* Cv, double* Cs);
 
+/* The same for the explosive point source (singleSource.cpp:29-127: Ricker moment rate of peak
+ * frequency fc, M0 = 1e16 N m, over 7^3 Gaussian weights): the rule above applied to the
+ * equivalent table, one node per stencil point with rate = s(t) * weight / mu_src on xx, yy, zz.
+ * mu_src: shear modulus at the source (or the smallest of the medium), mu_max the largest.
+ * No counterpart in the reference. */
+int cgfdm_point_source_scales(double DH, double fc, double mu_src, double mu_max, double* Cv, double* Cs);
+
 /* Uploads the model and builds the packed metric/Jacobian/medium array in FP64 on the GPU,
  * the free-surface rDZ matrices and the PML profiles.
  *  terrain : height (m) of the surface on the padded (NY+6)x(NX+6) grid, x fastest;
@@ -332,7 +339,8 @@ int cgfdm_server_wait(cgfdm_server* s, long long* ids, int max, int timeout_ms);
 
 /* Copies a completed request's seismograms (nst x NT x 9, as cgfdm_get_stations) and PGV
  * (NY x NX x 2) out (either may be NULL), fills info, releases the record and returns the
- * request's status. */
+ * request's status: CGFDM_ENONFINITE (outputs still copied) when a seismogram or PGV value is not
+ * finite. */
 int cgfdm_server_result(cgfdm_server* s, long long id, float* stations, float* pgv, cgfdm_request_info* info);
 
 /* Counters of the server. */

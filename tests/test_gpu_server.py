@@ -73,11 +73,13 @@ def _serve(cg, reqs, prec, **kw):
 
 def _served_scales(cg, r):
     """The scales the server chooses for a 16-bit request without its own (cgfdm_source_scales
-    with the largest shear modulus of the layers and of the default medium)."""
+    with the largest shear modulus of the layers and of the default medium; a point source:
+    cgfdm_point_source_scales with the smallest shear modulus of the layers)."""
     tops, vs, vp, rho = r["layers"]
-    mu = max(float((np.asarray(rho, np.float64) * np.asarray(vs, np.float64) ** 2).max()), 2670.0 * 3464.0 ** 2)
+    mus = np.asarray(rho, np.float64) * np.asarray(vs, np.float64) ** 2
+    mu = max(float(mus.max()), 2670.0 * 3464.0 ** 2)
     if "point" in r:
-        return cg.source_scales(r["desc"].DH)
+        return cg.point_source_scales(r["desc"].DH, r["point"][3], float(mus.min()), mu)
     xyz, tab, dt = r["table"]
     return cg.source_scales(r["desc"].DH, xyz, tab, dt, mu)
 
@@ -249,3 +251,25 @@ def test_admission_order(gpu, least_work_first):
         assert infos["small"]["t_admit"] < infos["big"]["t_admit"]
     else:
         assert infos["big"]["t_admit"] < infos["small"]["t_admit"]
+
+
+def test_served_non_finite_result_is_reported(gpu):
+    """A 16-bit request whose own scales put its field out of range (Cv 1e9: velocities of mm/s
+    overflow FP16) retires with CGFDM_ENONFINITE and its outputs still copied; a request served
+    beside it is unaffected."""
+    cg = gpu
+    bad, good = _requests(cg, "fp16", seed=5, n=2, grid=(40, 38, 30))
+    bad["desc"].Cv, bad["desc"].Cs = 1e9, 1e9 / 5e9
+    with cg.Server(precision="fp16", max_domains=4) as s:
+        ids = {s.submit(bad): "bad", s.submit(good): "good"}
+        seen = {}
+        while len(seen) < 2:
+            got = s.wait(timeout_ms=60000)
+            assert got, f"no request completed within 60 s ({s.stats()})"
+            for rid in got:
+                seen[ids[rid]] = s.result(rid)
+    rc, st, pgv, info = seen["bad"]
+    assert rc == cg.STATUS_CODES["ENONFINITE"] and "non-finite" in info["error"]
+    assert not np.isfinite(st).all()
+    rc, st, pgv, info = seen["good"]
+    assert rc == 0 and np.isfinite(st).all() and np.isfinite(pgv).all()
