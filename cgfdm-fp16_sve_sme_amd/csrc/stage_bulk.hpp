@@ -230,6 +230,13 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
     const int nk = ke - kb;
     const int k0 = FB3 > 0 ? kb : ke - 1;
     const int kmax = L.nz_ - 1;
+    // The loop runs a multiple of 5 iterations; the ones past the chunk store nothing and load
+    // planes an active iteration has just loaded (served by L2): the last centre plane, and for
+    // the queue the last plane an active iteration's stencil reaches (requested at iteration
+    // nk - 2). An iteration requests the next queue plane and the next plane's halo and metric
+    // only for an active successor.
+    const int klast = min(max(k0 + (nk - 1) * FB3, 0), kmax);
+    const int kqlast = min(max(k0 + (nk + 2) * FB3, 0), kmax);
 
     const char* __restrict__ Wb = (const char*)p.Win;
     const char* __restrict__ Cb = (const char*)p.cjm;
@@ -361,8 +368,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         const W9<S>& q3 = q[(3 + R) % 5];
         const W9<S>& q4 = q[(4 + R) % 5];
         const bool active = it < nk;  // iterations past the chunk store nothing
+        const bool next = it + 1 < nk;
         auto* Lb = lds[it & 1];
-        const int kc = min(max(k, 0), kmax);  // clamped plane (only the inactive iterations need it)
+        const int kc = active ? min(max(k, 0), kmax) : klast;
         const long long pw = (long long)kc * wpl;
         if constexpr (!PREFETCH) fetch_plane(kc, hv, M, Mb);
         // stage the centre plane and its halo in LDS
@@ -389,9 +397,9 @@ __device__ __forceinline__ void bulk_tile(const StageParams& p, int FB1, int FB2
         // requests for later iterations: the plane entering the queue and, with prefetch, the
         // next plane's halo and metric/medium pairs (clamped at the grid end; unused there)
         W9<S> nq;
-        nq.template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)min(max(k + 4 * FB3, 0), kmax) * wpl), wrange), col,
-                             col1);
-        if constexpr (PREFETCH) fetch_plane(min(max(k + FB3, 0), kmax), hvN, MN, MbN);
+        const int kq = next ? min(max(k + 4 * FB3, 0), kmax) : kqlast;
+        nq.template load<PB, CPW>(make_rsrc((const S*)(Wb + (long long)kq * wpl), wrange), col, col1);
+        if constexpr (PREFETCH) fetch_plane(next ? min(max(k + FB3, 0), kmax) : klast, hvN, MN, MbN);
         // RK registers of this plane: needed only by the final update
         const Rsrc rT = make_rsrc((const S*)(Tb + pw), wrange);
         W9<S> ra, rt;

@@ -112,6 +112,13 @@ inline void stage_boxes(StageParams& q, int use_fast) {
     }
 }
 
+// A z-chunk length for the tile kernels, whose loop runs a multiple of 5 iterations: kc rounded
+// up to a multiple of 5 (at most nzb), so only the last chunk of a box has idle iterations. Used
+// for the bulk box; the PML boxes keep more, shorter chunks (their idle iterations load only
+// planes L2 holds, and a domain served alone needs the parallelism: rounding their chunks up
+// served one request at a time 11 % slower).
+inline int chunk5(int kc, int nzb) { return std::max(1, std::min(nzb, (kc + 4) / 5 * 5)); }
+
 // Tiles of every PML box, grouped by kind (pml_mask_at order); z chunks sized so each kind has
 // about `want` tiles (a single domain: enough workgroups to fill the GPU; one of many batched
 // domains: its share), with chunks of at least 5 planes (the tile loop's unit): the small edge
@@ -170,9 +177,11 @@ inline BulkGeom bulk_geom(const StageParams& q) {
     }
     kc = std::max(1, std::min(kc, nzb));
     // a requested length is the longest chunk: the planes are split evenly over as many chunks
-    // (a short last chunk would pay the queue's start-up planes for little work)
+    // (a short last chunk would pay the queue's start-up planes for little work), then the length
+    // is rounded up to a multiple of 5 (chunk5)
     g.nch = (nzb + kc - 1) / kc;
-    g.kc = (nzb + g.nch - 1) / g.nch;
+    g.kc = chunk5((nzb + g.nch - 1) / g.nch, nzb);
+    g.nch = (nzb + g.kc - 1) / g.kc;
     return g;
 }
 
