@@ -90,3 +90,44 @@ def test_bad_arguments_are_rejected(gpu):
                                             ctypes.byref(cs)) == -1
     zero = np.zeros((4, 1, 6), np.float32)  # no source at all: the reference scales
     assert cg.source_scales(100.0, x, zero, 0.01, 1e10) == pytest.approx((5e3, 1e-6))
+
+
+def test_point_source_scales_follow_the_wavelet_and_the_medium(gpu):
+    """cgfdm_point_source_scales: the table rule applied to the point source's equivalent table
+    (the Ricker moment rate of peak frequency fc over the 7^3 Gaussian weights, divided by the
+    shear modulus at the source). Velocity-bound for these inputs, so the scale is proportional to
+    mu_src and DH^2 and inversely to fc (the peak moment rate is proportional to fc), with the
+    reference's ratio Cv/Cs = 5e9."""
+    cg = gpu
+    mu = 2670.0 * 3464.0 ** 2
+    cv, cs = cg.point_source_scales(100.0, 2.0, mu, mu)
+    assert cv / cs == pytest.approx(5e9, rel=1e-12)
+    assert cg.point_source_scales(100.0, 2.0, mu / 2, mu)[0] == pytest.approx(cv / 2, rel=1e-5)
+    assert cg.point_source_scales(200.0, 2.0, mu, mu)[0] == pytest.approx(4 * cv, rel=1e-5)
+    assert cg.point_source_scales(100.0, 4.0, mu, mu)[0] == pytest.approx(cv / 2, rel=1e-3)
+    with pytest.raises(cg.CgfdmError):
+        cg.point_source_scales(100.0, 0.0, mu, mu)
+
+
+def test_point_source_scales_bound_the_field_of_a_soft_medium(gpu, oracle_lib):
+    """In a soft medium (Vs 1800 m/s), the FP32 oracle's peak particle velocity times the chosen
+    Cv stays below the rule's target 1e4 (the estimate is an upper bound; 4.4e3 here), while the
+    reference's fixed Cv = 5e3 would put it past the FP16 maximum (65504: 14.1 m/s x 5e3)."""
+    cg, po = gpu, oracle_lib
+    NX, NY, NZ = 40, 40, 36
+    vs = np.full((NZ, NY, NX), 1800.0, np.float32)
+    vp = (1.73 * vs).astype(np.float32)
+    rho = (1740.0 * (vp / 1000.0) ** 0.25).astype(np.float32)
+    d = cg.make_desc(NX, NY, NZ, DH=100.0, DT=0.008, NT=160, nPML=6)
+    o = po.Oracle(d, po.F32)
+    o.set_model(None, vs, vp, rho)
+    o.set_source_point(NX // 2, NY // 2, NZ // 2, 2.5)
+    peak = 0.0
+    for _ in range(16):
+        o.step(10)
+        peak = max(peak, float(np.abs(o.state()[:3]).max()))
+    o.close()
+    mu = float(rho[0, 0, 0]) * 1800.0 ** 2
+    cv, _ = cg.point_source_scales(100.0, 2.5, mu, mu)
+    assert peak * cv < 1e4
+    assert peak * 5e3 > 65504.0
