@@ -492,6 +492,13 @@ int cgfdm_domain_create(const cgfdm_desc* desc, int device, cgfdm_domain** out) 
         cgfdm_domain_destroy(h);
         return CGFDM_EHIP;
     }
+    for (int q = 0; q < 2; ++q)
+        if ((e = hipStreamCreate(&h->side2[q])) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&h->evJoin2[q], hipEventDisableTiming)) != hipSuccess) {
+            set_err("side stream: %s", hipGetErrorString(e));
+            cgfdm_domain_destroy(h);
+            return CGFDM_EHIP;
+        }
     if ((rc = domain_init(h, desc, device))) {
         cgfdm_domain_destroy(h);
         return rc;
@@ -510,6 +517,13 @@ int cgfdm_domain_destroy(cgfdm_domain* h) {
     if (h->side) {
         (void)hipStreamSynchronize(h->side);
         (void)hipStreamDestroy(h->side);
+    }
+    for (int q = 0; q < 2; ++q) {
+        if (h->side2[q]) {
+            (void)hipStreamSynchronize(h->side2[q]);
+            (void)hipStreamDestroy(h->side2[q]);
+        }
+        if (h->evJoin2[q]) (void)hipEventDestroy(h->evJoin2[q]);
     }
     if (h->evFork) (void)hipEventDestroy(h->evFork);
     if (h->evJoin) (void)hipEventDestroy(h->evJoin);
@@ -699,8 +713,12 @@ int cgfdm_step(cgfdm_domain* h, int nsteps) {
                     h->evUsed = 0;
                 }
             } else {
-                // untimed: the boundary kernels run beside the bulk kernel on the side stream
-                const StageFork fk{h->side, h->evFork, h->evJoin};
+                // untimed: the boundary kernels run beside the bulk kernel on the side streams
+                StageFork fk{h->side, h->evFork, h->evJoin};
+                for (int q = 0; q < 2; ++q) {
+                    fk.side2[q] = h->side2[q];
+                    fk.join2[q] = h->evJoin2[q];
+                }
                 LCHK(launch_stage(h->prec, p, 1, h->stream, nullptr, h->side ? &fk : nullptr));
             }
             FB1 = -FB1; FB2 = -FB2; FB3 = -FB3;

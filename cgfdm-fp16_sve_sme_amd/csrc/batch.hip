@@ -249,29 +249,45 @@ int batch_clear_slot(const BatchDev& b, int s, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-// One RK stage of every resident domain: the bulk tiles on st, the PML, free-surface and general
-// work on the side stream of fk (if given) concurrently, joined back into st.
+// One RK stage of every resident domain: the bulk tiles on st; the PML, free-surface and general
+// work on the side stream(s) of fk (if given) concurrently, joined back into st. With three side
+// streams: the x faces on the first, the y and bottom faces on the second, edges, corners, the
+// free-surface layers and the general boxes on the third, so the boundary kernels of small
+// domains (a large share of their points) overlap each other instead of running one by one.
 template <class S, int TX, int TY, int IRK>
 static int batch_stage(const BatchDev& b, const BatchGrid& g, hipStream_t st, const StageFork* fk) {
-    hipStream_t sb = st;
+    hipStream_t s0 = st, s1 = st, s2 = st;
+    const bool three = fk && fk->side && fk->side2[0] && fk->side2[1];
     if (fk && fk->side) {
         if (hipEventRecord(fk->fork, st) != hipSuccess || hipStreamWaitEvent(fk->side, fk->fork, 0) != hipSuccess)
             return -3;
-        sb = fk->side;
+        s0 = s1 = s2 = fk->side;
+        if (three) {
+            for (int q = 0; q < 2; ++q)
+                if (hipStreamWaitEvent(fk->side2[q], fk->fork, 0) != hipSuccess) return -3;
+            s1 = fk->side2[0];
+            s2 = fk->side2[1];
+        }
     }
     k_bulk_multi<S, TX, TY, IRK><<<g.bulkBlocks, TX * TY, 0, st>>>(b);
-    k_pml_multi<S, IRK, 1><<<g.pmlBlocks[0], pml_tx(1) * pml_ty(1), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 2><<<g.pmlBlocks[1], pml_tx(2) * pml_ty(2), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 4><<<g.pmlBlocks[2], pml_tx(4) * pml_ty(4), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 3><<<g.pmlBlocks[3], pml_tx(3) * pml_ty(3), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 5><<<g.pmlBlocks[4], pml_tx(5) * pml_ty(5), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 6><<<g.pmlBlocks[5], pml_tx(6) * pml_ty(6), 0, sb>>>(b);
-    k_pml_multi<S, IRK, 7><<<g.pmlBlocks[6], pml_tx(7) * pml_ty(7), 0, sb>>>(b);
-    k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, sb>>>(b);
-    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, sb>>>(b, IRK);
-    if (sb != st) {
-        if (hipEventRecord(fk->join, sb) != hipSuccess || hipStreamWaitEvent(st, fk->join, 0) != hipSuccess)
+    k_pml_multi<S, IRK, 1><<<g.pmlBlocks[0], pml_tx(1) * pml_ty(1), 0, s0>>>(b);
+    k_pml_multi<S, IRK, 2><<<g.pmlBlocks[1], pml_tx(2) * pml_ty(2), 0, s1>>>(b);
+    k_pml_multi<S, IRK, 4><<<g.pmlBlocks[2], pml_tx(4) * pml_ty(4), 0, s1>>>(b);
+    k_pml_multi<S, IRK, 3><<<g.pmlBlocks[3], pml_tx(3) * pml_ty(3), 0, s2>>>(b);
+    k_pml_multi<S, IRK, 5><<<g.pmlBlocks[4], pml_tx(5) * pml_ty(5), 0, s2>>>(b);
+    k_pml_multi<S, IRK, 6><<<g.pmlBlocks[5], pml_tx(6) * pml_ty(6), 0, s2>>>(b);
+    k_pml_multi<S, IRK, 7><<<g.pmlBlocks[6], pml_tx(7) * pml_ty(7), 0, s2>>>(b);
+    k_fs_multi<S, IRK><<<g.fsBlocks, kFsTx * kFsTy, 0, s2>>>(b);
+    k_general_multi<S><<<g.genBlocks, kGenChunk, 0, s2>>>(b, IRK);
+    if (s0 != st) {
+        if (hipEventRecord(fk->join, s0) != hipSuccess || hipStreamWaitEvent(st, fk->join, 0) != hipSuccess)
             return -3;
+    }
+    if (three) {
+        const hipStream_t sq[2] = {s1, s2};
+        for (int q = 0; q < 2; ++q)
+            if (hipEventRecord(fk->join2[q], sq[q]) != hipSuccess || hipStreamWaitEvent(st, fk->join2[q], 0) != hipSuccess)
+                return -3;
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }

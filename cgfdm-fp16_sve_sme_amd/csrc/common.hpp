@@ -249,7 +249,15 @@ int launch_hmin(const SetupParams& q, unsigned long long* out, hipStream_t s);
 // general kernels run on it concurrently with the bulk kernel (they write disjoint points and
 // read only the stage input), joined back into s at the end; mid (if given) is recorded on s
 // after the bulk kernel.
-struct StageFork { hipStream_t side; hipEvent_t fork, join; };
+// The serving engine may also give two more side streams (side2, with their join events): the
+// boundary kernels are then spread over three side streams, which run concurrently with each
+// other as well (every kernel of a stage writes its own points and their auxiliaries only).
+struct StageFork {
+    hipStream_t side;
+    hipEvent_t fork, join;
+    hipStream_t side2[2] = {nullptr, nullptr};
+    hipEvent_t join2[2] = {nullptr, nullptr};
+};
 int launch_stage(int prec, const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid,
                  const StageFork* fk = nullptr);
 

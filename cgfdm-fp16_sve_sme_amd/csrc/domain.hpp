@@ -47,6 +47,8 @@ struct cgfdm_domain {
     // side stream of the boundary kernels and its fork/join events (own-stream domains)
     hipStream_t side = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
+    hipStream_t side2[2] = {nullptr, nullptr};  // two more side streams for the boundary kernels
+    hipEvent_t evJoin2[2] = {nullptr, nullptr};
     bool async = false;       // pooled mode: no host synchronisation in set-up calls
     cg::Bump devPool, hostPool;
     bool pooled = false;

@@ -175,10 +175,19 @@ struct cgfdm_server {
     BatchGrid grid{};
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    // side stream of the boundary kernels, forked from and joined into `stream` every stage
+    // side streams of the boundary kernels, forked from and joined into `stream` every stage
     hipStream_t side = nullptr;
+    hipStream_t side2[2] = {nullptr, nullptr};
     hipEvent_t evFork = nullptr, evJoin = nullptr;
-    StageFork fork() const { return StageFork{side, evFork, evJoin}; }
+    hipEvent_t evJoin2[2] = {nullptr, nullptr};
+    StageFork fork() const {
+        StageFork f{side, evFork, evJoin};
+        for (int q = 0; q < 2; ++q) {
+            f.side2[q] = side2[q];
+            f.join2[q] = evJoin2[q];
+        }
+        return f;
+    }
     Clock::time_point t0;
 
     std::mutex mu;
@@ -514,6 +523,10 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         if (S->evFork) (void)hipEventDestroy(S->evFork);
         if (S->evJoin) (void)hipEventDestroy(S->evJoin);
         if (S->side) (void)hipStreamDestroy(S->side);
+        for (int q = 0; q < 2; ++q) {
+            if (S->evJoin2[q]) (void)hipEventDestroy(S->evJoin2[q]);
+            if (S->side2[q]) (void)hipStreamDestroy(S->side2[q]);
+        }
         (void)hipStreamDestroy(S->stream);
         return rc;
     };
@@ -523,6 +536,12 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
         set_err("side stream: %s", hipGetErrorString(e));
         return undo(CGFDM_EHIP);
     }
+    for (int q = 0; q < 2; ++q)
+        if ((e = hipStreamCreate(&S->side2[q])) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&S->evJoin2[q], hipEventDisableTiming)) != hipSuccess) {
+            set_err("side stream: %s", hipGetErrorString(e));
+            return undo(CGFDM_EHIP);
+        }
     if ((e = hipMalloc((void**)&S->devBase, budget)) != hipSuccess) { (void)hipGetLastError(); set_err("arena of %zu bytes: %s", budget, hipGetErrorString(e)); return undo(CGFDM_ENOMEM); }
     if ((e = hipHostMalloc((void**)&S->hostBase, hostBudget, 0)) != hipSuccess) { set_err("pinned arena: %s", hipGetErrorString(e)); return undo(CGFDM_ENOMEM); }
     S->devArena.reset(budget);
@@ -742,9 +761,14 @@ int cgfdm_server_destroy(cgfdm_server* s) {
     (void)hipFree(s->b.tab);
     for (auto& v : s->ev) (void)hipEventDestroy(v);
     (void)hipStreamSynchronize(s->side);
+    for (int q = 0; q < 2; ++q) (void)hipStreamSynchronize(s->side2[q]);
     (void)hipEventDestroy(s->evFork);
     (void)hipEventDestroy(s->evJoin);
     (void)hipStreamDestroy(s->side);
+    for (int q = 0; q < 2; ++q) {
+        (void)hipEventDestroy(s->evJoin2[q]);
+        (void)hipStreamDestroy(s->side2[q]);
+    }
     (void)hipStreamDestroy(s->stream);
     delete s;
     return CGFDM_OK;

@@ -36,13 +36,20 @@ template <class S>
 static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hipEvent_t mid, const StageFork* fk) {
     StageParams q = p;
     stage_boxes(q, use_fast);
-    // boundary kernels on the side stream, after everything enqueued on s so far
+    // boundary kernels on the side stream(s), after everything enqueued on s so far: with three,
+    // the x faces on the first, the y and bottom faces on the second, edges, corners, the
+    // free-surface layers and the general boxes on the third
     const bool forked = fk && fk->side && use_fast != 2;
+    const bool three = forked && fk->side2[0] && fk->side2[1];
     if (forked) {
         if (hipEventRecord(fk->fork, s) != hipSuccess || hipStreamWaitEvent(fk->side, fk->fork, 0) != hipSuccess)
             return -3;
+        if (three)
+            for (int w = 0; w < 2; ++w)
+                if (hipStreamWaitEvent(fk->side2[w], fk->fork, 0) != hipSuccess) return -3;
     }
     const hipStream_t sb = forked ? fk->side : s;
+    const hipStream_t s1 = three ? fk->side2[0] : sb, s2 = three ? fk->side2[1] : sb;
     if (q.bulk_on) {
         const BulkGeom bg = bulk_geom(q);
         const int cfg = bg.cfg, kc = bg.kc, gx = bg.gx, gy = bg.gy;
@@ -59,27 +66,32 @@ static int launch_stage_t(const StageParams& p, int use_fast, hipStream_t s, hip
     if (use_fast != 2) {
         const PmlTiles t = pml_tiles(q, 1024);
         launch_pml<S, 1>(q, t, sb);
-        launch_pml<S, 2>(q, t, sb);
-        launch_pml<S, 4>(q, t, sb);
-        launch_pml<S, 3>(q, t, sb);
-        launch_pml<S, 5>(q, t, sb);
-        launch_pml<S, 6>(q, t, sb);
-        launch_pml<S, 7>(q, t, sb);
+        launch_pml<S, 2>(q, t, s1);
+        launch_pml<S, 4>(q, t, s1);
+        launch_pml<S, 3>(q, t, s2);
+        launch_pml<S, 5>(q, t, s2);
+        launch_pml<S, 6>(q, t, s2);
+        launch_pml<S, 7>(q, t, s2);
         if (hipGetLastError() != hipSuccess) return -3;
     }
     if (q.fs_on && use_fast != 2) {
         const int gx = (q.bi1 - q.bi0 + kFsTx - 1) / kFsTx, gy = (q.bj1 - q.bj0 + kFsTy - 1) / kFsTy;
-        k_stage_fs<S, kFsTx, kFsTy><<<(unsigned)(gx * gy), kFsTx * kFsTy, 0, sb>>>(q, gx, gy);
+        k_stage_fs<S, kFsTx, kFsTy><<<(unsigned)(gx * gy), kFsTx * kFsTy, 0, s2>>>(q, gx, gy);
     }
     const long long n = q.boxStart[q.nbox];
     if (n > 0 && use_fast != 2) {
         const int bs = 256;
-        k_stage_general<S><<<(unsigned)((n + bs - 1) / bs), bs, 0, sb>>>(q);
+        k_stage_general<S><<<(unsigned)((n + bs - 1) / bs), bs, 0, s2>>>(q);
     }
     if (hipGetLastError() != hipSuccess) return -3;
     if (forked) {
         if (hipEventRecord(fk->join, fk->side) != hipSuccess || hipStreamWaitEvent(s, fk->join, 0) != hipSuccess)
             return -3;
+        if (three)
+            for (int w = 0; w < 2; ++w)
+                if (hipEventRecord(fk->join2[w], fk->side2[w]) != hipSuccess ||
+                    hipStreamWaitEvent(s, fk->join2[w], 0) != hipSuccess)
+                    return -3;
     }
     return 0;
 }
