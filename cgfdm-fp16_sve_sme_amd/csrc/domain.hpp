@@ -112,20 +112,28 @@ namespace cg {
 // Reference scale factors of 16-bit storage (macro.h:65-73): Cv/Cs = 5e9 fixes the stored medium
 // (rho*Cs/Cv), kept by every choice below.
 constexpr double kRefCv = 5.0e3, kRefCs = 1.0e-6, kRefCrho = 1.0e6;
-// Targets of the scaled peak particle velocity and stress that source_scales aims at: the
-// near-field estimates below exceed the measured peaks (by ~1.6x on the throughput workload),
-// and 1e4 leaves a further 6x below the FP16 maximum (65504).
-constexpr double kVelTarget = 1.0e4, kStressTarget = 1.0e4;
+// Targets of the scaled peak particle velocity and stress that source_scales aims at. The stored
+// state is not the largest stored number: next to a single-node moment source the RK stage values
+// of the velocity (W = m + alpha h) overshoot the state's peak, by up to ~4.5x for a slow source
+// (Ricker fc 0.5 Hz, Vs 4000 m/s, DT 0.008 s, DH 100 m: the FP16 oracle goes non-finite once the
+// stored peak passes ~1.4e4), so the velocity target leaves 16x below the FP16 maximum (65504).
+constexpr double kVelTarget = 4.0e3, kStressTarget = 1.0e4;
+// Peak velocity one cell from a single node of moment rate M' in the discrete scheme, in units of
+// the continuum's M' / (4 pi mu DH^2): 1.7 to 2.3 measured with the FP32 oracle over 19 double
+// couples (fc 0.6 to 3 Hz, Vs 3400 m/s); the rule takes 2.5.
+constexpr double kNodeField = 2.5;
 
 // Scale factors (Cv, Cs) of 16-bit storage chosen from the source. Without a table (rate ==
 // null): the reference's (5e3, 1e-6) times (DH/100)^2 (point_source_scales below estimates the
 // point source from its wavelet and the medium instead). Moment-rate
 // table (nt x npts x 6, tensor * area * slip rate, as cgfdm_set_source_table takes it, at grid
 // nodes xyz): the peak particle velocity is estimated as the largest, over nodes p and radii
-// R = DH, 2 DH, ..., 16 DH, of 2 * sum_{|q - p| <= R} max_t|rate_q| / (4 pi R^2). At R = DH this
-// is the near field of one node's moment rate M' = mu * rate one cell away, M'/(4 pi mu DH^2)
-// (mu cancels); at larger R it is the field of a patch of nodes slipping together, which tends to
-// the slip rate / 2 of a planar fault (each node holding area * slip rate). The peak stress is
+// R = DH, 2 DH, ..., 16 DH, of 2 * sum_{|q - p| <= R} max_t|rate_q| / (4 pi R^2), except at
+// R = DH: (kNodeField * max_t|rate_p| + sum_{0 < |q - p| <= DH} max_t|rate_q|) / (4 pi DH^2).
+// That is the near field of one node's moment rate M' = mu * rate one cell away,
+// M'/(4 pi mu DH^2) (mu cancels), with the discrete scheme's excess for the node itself; at
+// larger R it is the field of a patch of nodes slipping together, which tends to the slip
+// rate / 2 of a planar fault (each node holding area * slip rate). The peak stress is
 // estimated from the static near field, mu_max * max_t|sum rate dt| / (4 pi DH^3). Cv is the
 // larger value that keeps both scaled estimates below their targets, Cs = Cv / 5e9.
 inline void source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, double dt,
@@ -161,9 +169,12 @@ inline void source_scales(double DH, const int* xyz, const float* rate, long lon
             for (int r = 0; r < 5; ++r)
                 if (d2 <= (double)(1 << r) * (1 << r)) sum[r] += peak[q];
         }
-        for (int r = 0; r < 5; ++r) {
+        // R = DH: the node's own rate with the discrete excess, its neighbours' without (a
+        // smooth distribution of nodes follows the continuum)
+        v = std::max(v, (sum[0] + (kNodeField - 1.0) * peak[p]) / (pi4 * DH * DH));
+        for (int r = 1; r < 5; ++r) {
             const double R = DH * (1 << r);
-            v = std::max(v, (r == 0 ? 1.0 : 2.0) * sum[r] / (pi4 * R * R));
+            v = std::max(v, 2.0 * sum[r] / (pi4 * R * R));
         }
     }
     const double T = mu_max * mmax / (pi4 * DH * DH * DH);

@@ -94,11 +94,14 @@ long long cgfdm_estimate_bytes(const cgfdm_desc* desc);
  *  rate = a moment-rate table as cgfdm_set_source_table takes it (nt x npts x 6, tensor * area *
  *      slip rate, at the global interior indices xyz[3*npts]) sampled every dt s, mu_max the
  *      largest shear modulus (Pa) of the medium: the peak velocity is estimated as the largest,
- *      over nodes p and radii R = DH .. 16 DH, of (1 at R = DH, else 2) * sum over the nodes
- *      within R of max_t|rate| / (4 pi R^2) (the near field of one node's moment rate; the
- *      field of a patch of nodes slipping together, which tends to half the slip rate of a
- *      planar fault), the peak stress as mu_max * max_t|sum rate dt| / (4 pi DH^3);
- *      Cv = min(1e4 / velocity, 5e9 * 1e4 / stress).
+ *      over nodes p and radii R = DH .. 16 DH, of 2 * sum over the nodes within R of
+ *      max_t|rate| / (4 pi R^2) (the field of a patch of nodes slipping together, which tends
+ *      to half the slip rate of a planar fault), at R = DH of (2.5 max_t|rate_p| + the sum over
+ *      p's neighbours) / (4 pi DH^2) (the near field of one node's moment rate, with the
+ *      discrete scheme's excess over the continuum for the node itself), the peak stress as
+ *      mu_max * max_t|sum rate dt| / (4 pi DH^3); Cv = min(4e3 / velocity, 5e9 * 1e4 / stress).
+ *      The velocity target leaves room for the RK stage values, which overshoot the stored
+ *      peak by up to ~4.5x next to a slow single-node source.
  * No counterpart in the reference, whose scales are compile-time constants. */
 int cgfdm_source_scales(double DH, const int* xyz, const float* rate, long long npts, int nt, float dt,
                         double mu_max, double* Cv, double* Cs);
