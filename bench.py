@@ -220,7 +220,7 @@ def serving(n, seed, max_domains, steps_per_launch, use_graph=True, modes=("batc
         out[name] = {"solves_per_s": round(n / span, 4), "p50_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 2),
                      "glups": round(lups / span / 1e9, 3), "peak_resident": st["peak_resident"],
-                     "launches": st["launches"],
+                     "launches": st["launches"], "rescaled": st["rescaled"],
                      "queue_p50_s": round(float(np.percentile(qt, 50)), 3),
                      "queue_p99_s": round(float(np.percentile(qt, 99)), 3),
                      "service_p50_s": round(float(np.percentile(sv, 50)), 3),

@@ -344,7 +344,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
                 ("hbm_budget", ctypes.c_longlong), ("host_budget", ctypes.c_longlong),
                 ("max_domains", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("admit_order", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("admit_order", ctypes.c_int),
+                ("rescale_retries", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -364,7 +365,8 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("NT", ctypes.c_int), ("nst", ctypes.c_int),
                 ("NX", ctypes.c_int), ("NY", ctypes.c_int),
                 ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
-                ("device_bytes", ctypes.c_longlong), ("error", ctypes.c_char * 160)]
+                ("device_bytes", ctypes.c_longlong), ("error", ctypes.c_char * 160),
+                ("Cv", ctypes.c_double), ("Cs", ctypes.c_double), ("rescaled", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -375,21 +377,23 @@ class ServerStats(ctypes.Structure):
                 ("graph", ctypes.c_int), ("tile_cfg", ctypes.c_int), ("bulk_blocks", ctypes.c_int),
                 ("general_blocks", ctypes.c_int),
                 ("arena_bytes", ctypes.c_longlong), ("arena_used", ctypes.c_longlong),
-                ("arena_peak", ctypes.c_longlong)]
+                ("arena_peak", ctypes.c_longlong), ("rescaled", ctypes.c_longlong)]
 
 
 class Server:
     """Continuous-batching server (cgfdm_server): submit requests, wait for completions, fetch
     results. Requests are dicts with keys desc, and either layers=(depth, vs, vp, rho) or
     medium=(vs, vp, rho); terrain (with desc.use_terrain); point=(X, Y, Z, fc) or
-    table=(xyz, rate[nt][npts][6], dt); stations (n x 3)."""
+    table=(xyz, rate[nt][npts][6], dt); stations (n x 3). With 16-bit storage a request whose
+    results come back non-finite is run again with 8x smaller scales, up to rescale_retries
+    times (0: reported at once as CGFDM_ENONFINITE)."""
 
     def __init__(self, precision=FP16, device=0, hbm_budget=0, host_budget=0, max_domains=16,
-                 steps_per_launch=8, use_graph=True, least_work_first=False):
+                 steps_per_launch=8, use_graph=True, least_work_first=False, rescale_retries=2):
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         c = ServerConfig(device, precision, hbm_budget, host_budget, max_domains, steps_per_launch,
-                         1 if use_graph else 0, 1 if least_work_first else 0)
+                         1 if use_graph else 0, 1 if least_work_first else 0, rescale_retries)
         h = ctypes.c_void_p()
         _chk(lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)))
         self.h = h
@@ -451,7 +455,8 @@ class Server:
         rc = lib().cgfdm_server_result(self.h, rid, _p(st), _p(pgv), ctypes.byref(info))
         i = {"status": info.status, "t_submit": info.t_submit, "t_admit": info.t_admit,
              "t_done": info.t_done, "device_bytes": info.device_bytes,
-             "error": info.error.decode(errors="replace")}
+             "error": info.error.decode(errors="replace"), "Cv": info.Cv, "Cs": info.Cs,
+             "rescaled": info.rescaled}
         return rc, st, pgv, i
 
     def stats(self):

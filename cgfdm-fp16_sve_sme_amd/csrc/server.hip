@@ -113,6 +113,7 @@ struct Req {
     std::string err;
     std::vector<float> stations, pgv;
     size_t devBytes = 0, hostBytes = 0;
+    int rescaled = 0;  // runs repeated with smaller scales after non-finite results
 };
 
 struct Resident {
@@ -472,15 +473,29 @@ void cgfdm_server::loop() {
             bool finite = true;
             for (float x : r.stations) finite = finite && std::isfinite(x);
             for (float x : r.pgv) finite = finite && std::isfinite(x);
+            devArena.release(R.devOff, r.devBytes);
+            hostArena.release(R.hostOff, r.hostBytes);
+            freeSlots.push_back(R.slot);
+            delete R.dom;
+            if (!finite && r.desc.precision != CGFDM_FP32 && r.rescaled < cfg.rescale_retries) {
+                // 16-bit storage out of range: run the request again from its start with scales
+                // 8x smaller (Cv/Cs unchanged, so the stored medium is the same), ahead of the
+                // waiting requests
+                const double Cv = r.desc.Cv > 0 ? r.desc.Cv : kRefCv, Cs = r.desc.Cs > 0 ? r.desc.Cs : kRefCs;
+                r.desc.Cv = Cv / 8.0;
+                r.desc.Cs = Cs / 8.0;
+                ++r.rescaled;
+                ++stats.rescaled;
+                r.stations.clear();
+                r.pgv.clear();
+                pending.push_front(std::move(R.req));
+                continue;
+            }
             if (!finite) {
                 r.status = CGFDM_ENONFINITE;
                 r.err = "non-finite seismogram or PGV value (16-bit storage out of range?)";
             }
             r.tDone = Clock::now();
-            devArena.release(R.devOff, r.devBytes);
-            hostArena.release(R.hostOff, r.hostBytes);
-            freeSlots.push_back(R.slot);
-            delete R.dom;
             ++stats.completed;
             done[r.id] = std::move(R.req);
         }
@@ -502,6 +517,7 @@ int cgfdm_server_create(const cgfdm_server_config* c, cgfdm_server** out) {
     }
     if (c->steps_per_launch < 1 || c->steps_per_launch > 64) { set_err("steps_per_launch must be in [1, 64]"); return CGFDM_EINVAL; }
     if (c->admit_order != 0 && c->admit_order != 1) { set_err("admit_order must be 0 or 1"); return CGFDM_EINVAL; }
+    if (c->rescale_retries < 0 || c->rescale_retries > 8) { set_err("rescale_retries must be in [0, 8]"); return CGFDM_EINVAL; }
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail("hipSetDevice", e);
     auto S = std::make_unique<cgfdm_server>();
@@ -718,6 +734,10 @@ int cgfdm_server_result(cgfdm_server* s, long long id, float* stations, float* p
         info->t_done = secs(s->t0, r->tDone);
         info->device_bytes = (long long)r->devBytes;
         std::snprintf(info->error, sizeof(info->error), "%s", r->err.c_str());
+        const bool low = r->desc.precision != CGFDM_FP32;
+        info->Cv = r->desc.Cv > 0 ? r->desc.Cv : (low ? kRefCv : 1.0);
+        info->Cs = r->desc.Cs > 0 ? r->desc.Cs : (low ? kRefCs : 1.0);
+        info->rescaled = r->rescaled;
     }
     if (r->status == CGFDM_OK || r->status == CGFDM_ENONFINITE) {
         if (stations && !r->stations.empty()) std::memcpy(stations, r->stations.data(), r->stations.size() * 4);

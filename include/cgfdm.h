@@ -287,6 +287,9 @@ typedef struct cgfdm_server_config {
                                60 s go first, oldest first — a lower median latency when more
                                requests wait than slots, and a bound on how long a large
                                request is passed over                                       */
+    int rescale_retries;    /* 16-bit storage: how many times a request whose seismograms or
+                               PGV come back non-finite is run again from its start with Cv
+                               and Cs divided by 8 (0..8; 0: report CGFDM_ENONFINITE at once) */
 } cgfdm_server_config;
 
 typedef struct cgfdm_request {
@@ -321,12 +324,15 @@ typedef struct cgfdm_request_info {
     double t_submit, t_admit, t_done;  /* seconds since the server started (admit -1: never) */
     long long device_bytes;     /* arena bytes the request held                              */
     char error[160];
+    double Cv, Cs;              /* scale factors of the run that produced the results        */
+    int rescaled;               /* runs repeated with 8x smaller scales (rescale_retries)    */
 } cgfdm_request_info;
 
 typedef struct cgfdm_server_stats {
     long long submitted, completed, failed, launches;
     int resident, pending, peak_resident, graph, tile_cfg, bulk_blocks, general_blocks;
     long long arena_bytes, arena_used, arena_peak;
+    long long rescaled;         /* runs repeated with smaller scales after non-finite results */
 } cgfdm_server_stats;
 
 /* Creates a server and starts its worker thread. */
@@ -343,7 +349,8 @@ int cgfdm_server_wait(cgfdm_server* s, long long* ids, int max, int timeout_ms);
 /* Copies a completed request's seismograms (nst x NT x 9, as cgfdm_get_stations) and PGV
  * (NY x NX x 2) out (either may be NULL), fills info, releases the record and returns the
  * request's status: CGFDM_ENONFINITE (outputs still copied) when a seismogram or PGV value is not
- * finite. */
+ * finite after the rescaled runs the configuration allows (rescale_retries; info->Cv, Cs and
+ * rescaled describe the run that produced the outputs). */
 int cgfdm_server_result(cgfdm_server* s, long long id, float* stations, float* pgv, cgfdm_request_info* info);
 
 /* Counters of the server. */

@@ -26,20 +26,25 @@ def test_library_exports_every_declared_symbol(gpu):
     assert not missing, missing
 
 
-def test_desc_layout_matches_header(gpu, tmp_path):
-    fields = [f[0] for f in gpu.Desc._fields_]
+@pytest.mark.parametrize("cname,pyname", [("cgfdm_desc", "Desc"), ("cgfdm_server_config", "ServerConfig"),
+                                          ("cgfdm_request", "Request"), ("cgfdm_request_info", "RequestInfo"),
+                                          ("cgfdm_server_stats", "ServerStats")])
+def test_struct_layout_matches_header(gpu, tmp_path, cname, pyname):
+    """The binding's ctypes mirror of each struct has the header's size and field offsets."""
+    cls = getattr(gpu, pyname)
+    fields = [f[0] for f in cls._fields_]
     src = tmp_path / "layout.c"
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){",
-             'printf("size %zu\\n", sizeof(cgfdm_desc));']
-    lines += [f'printf("{f} %zu\\n", offsetof(cgfdm_desc, {f}));' for f in fields]
+             f'printf("size %zu\\n", sizeof({cname}));']
+    lines += [f'printf("{f} %zu\\n", offsetof({cname}, {f}));' for f in fields]
     lines += ["return 0;}"]
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
     subprocess.check_call(["gcc", "-o", str(exe), str(src)])
     out = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
-    assert int(out["size"]) == ctypes.sizeof(gpu.Desc)
+    assert int(out["size"]) == ctypes.sizeof(cls)
     for f in fields:
-        assert int(out[f]) == getattr(gpu.Desc, f).offset, f
+        assert int(out[f]) == getattr(cls, f).offset, f
 
 
 def _create_rc(gpu, desc):
@@ -73,6 +78,10 @@ def test_bad_descriptors_are_rejected(gpu):
     c = gpu.ServerConfig(0, gpu.FP16, 0, 0, 4, 8, 1, 2)
     assert gpu.lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)) == -1
     assert "admit_order" in gpu.lib().cgfdm_last_error().decode()
+    # and one with more rescaled runs than the server allows
+    c = gpu.ServerConfig(0, gpu.FP16, 0, 0, 4, 8, 1, 0, 9)
+    assert gpu.lib().cgfdm_server_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert "rescale_retries" in gpu.lib().cgfdm_last_error().decode()
 
 
 def test_estimate_bytes_scales_with_precision(gpu):

@@ -260,7 +260,7 @@ def test_served_non_finite_result_is_reported(gpu):
     cg = gpu
     bad, good = _requests(cg, "fp16", seed=5, n=2, grid=(40, 38, 30))
     bad["desc"].Cv, bad["desc"].Cs = 1e9, 1e9 / 5e9
-    with cg.Server(precision="fp16", max_domains=4) as s:
+    with cg.Server(precision="fp16", max_domains=4, rescale_retries=0) as s:
         ids = {s.submit(bad): "bad", s.submit(good): "good"}
         seen = {}
         while len(seen) < 2:
@@ -273,3 +273,38 @@ def test_served_non_finite_result_is_reported(gpu):
     assert not np.isfinite(st).all()
     rc, st, pgv, info = seen["good"]
     assert rc == 0 and np.isfinite(st).all() and np.isfinite(pgv).all()
+
+
+def test_non_finite_request_is_run_again_with_smaller_scales(gpu):
+    """With rescale_retries, the request of the test above (Cv 1e9) is run again from its start
+    with Cv and Cs divided by 8 until its results are finite: it completes with status OK, its
+    info names the scales of the run that produced the results and how many runs were repeated,
+    and the results are bit-identical to the same request served once with those scales."""
+    cg = gpu
+    bad, good = _requests(cg, "fp16", seed=5, n=2, grid=(40, 38, 30))
+    bad["desc"].Cv, bad["desc"].Cs = 1e9, 1e9 / 5e9
+    with cg.Server(precision="fp16", max_domains=4, rescale_retries=8) as s:
+        ids = {s.submit(bad): "bad", s.submit(good): "good"}
+        seen = {}
+        while len(seen) < 2:
+            got = s.wait(timeout_ms=60000)
+            assert got, f"no request completed within 60 s ({s.stats()})"
+            for rid in got:
+                seen[ids[rid]] = s.result(rid)
+        stats = s.stats()
+    rc, st, pgv, info = seen["bad"]
+    k = info["rescaled"]
+    print(f"rescaled {k} times, Cv {info['Cv']:.4g}")
+    assert rc == 0 and k >= 1 and stats["rescaled"] == k and stats["completed"] == 2
+    assert info["Cv"] == pytest.approx(1e9 / 8 ** k, rel=1e-12)
+    assert info["Cs"] == pytest.approx(1e9 / 5e9 / 8 ** k, rel=1e-12)
+    assert np.isfinite(st).all() and np.isfinite(pgv).all()
+    assert seen["good"][0] == 0 and seen["good"][3]["rescaled"] == 0
+    again, _ = _requests(cg, "fp16", seed=5, n=2, grid=(40, 38, 30))
+    again["desc"].Cv, again["desc"].Cs = info["Cv"], info["Cs"]
+    with cg.Server(precision="fp16", max_domains=4, rescale_retries=0) as s:
+        rid = s.submit(again)
+        assert s.wait(timeout_ms=60000) == [rid]
+        rc2, st2, pgv2, info2 = s.result(rid)
+    assert rc2 == 0 and info2["rescaled"] == 0
+    assert np.array_equal(st2, st) and np.array_equal(pgv2, pgv)
