@@ -469,7 +469,7 @@ void cgfdm_server::loop() {
             const size_t nst = r.st.size() / 3;
             r.stations.assign(R.resSt, R.resSt + nst * 9 * (size_t)r.desc.NT);
             r.pgv.assign(R.resPgv, R.resPgv + 2 * (size_t)r.desc.NX * r.desc.NY);
-            // non-finite guard of served requests: judged on the copied results
+            // non-finite guard of served requests: decided on the copied results
             bool finite = true;
             for (float x : r.stations) finite = finite && std::isfinite(x);
             for (float x : r.pgv) finite = finite && std::isfinite(x);
